@@ -1,0 +1,131 @@
+/*
+ * spk.h -- C ABI of the MI355X (gfx950) SPOOKY solver library (libspk.so).
+ *
+ * Plain C types only: handles, device pointers as void*, sizes, doubles.
+ * Every call is asynchronous on the context's HIP stream unless it says it
+ * synchronizes.  Every call returns SPK_OK (0) or a negative status; the text
+ * of the last error on the calling thread is in spk_last_error().  Unlike the
+ * reference's CUDA_RT_CALL/CUFFT_CALL (libs/cufft_utils.h:53-84), which print
+ * and continue, errors here are returned and nothing continues past them.
+ *
+ * Layouts are the reference's (common.hpp:12-14, parameters.cu:650):
+ *   spectral field : complex [NX][NY][NZ/2+1], kz fastest, interleaved re/im
+ *   real field     : [NX][NY][2*(NZ/2+1)] (2 padding values per row)
+ *   field set      : field n at offset n*NTC complex, NTC = NX*NY*(NZ/2+1)
+ *   field order    : vx,vy,vz,bx,by,bz,th as present (parameters.cu:568-640)
+ * The solver state is FP64 complex.  FFTs and real-space products run in the
+ * solver's "bulk" precision (SPK_F64, or SPK_F32 for the FP32-bulk mode).
+ */
+#ifndef SPK_H
+#define SPK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPK_ABI_VERSION 1
+
+enum spk_status {
+    SPK_OK = 0,
+    SPK_ERR_ARG = -1,         /* bad argument or unsupported shape */
+    SPK_ERR_HIP = -2,         /* HIP runtime error */
+    SPK_ERR_ALLOC = -3,       /* device allocation failed */
+    SPK_ERR_STATE = -4        /* call not valid for this solver configuration */
+};
+
+enum spk_precision { SPK_F64 = 0, SPK_F32 = 1 };
+
+int spk_abi_version(void);
+const char *spk_last_error(void);
+
+/* ---- context: device + stream (replaces the default-stream globals) ---- */
+typedef struct spk_ctx_s *spk_ctx;
+/* stream: a hipStream_t, or NULL for the null stream */
+int spk_ctx_create(int device, void *stream, spk_ctx *out);
+int spk_ctx_destroy(spk_ctx ctx);
+int spk_ctx_sync(spk_ctx ctx); /* synchronizes the context stream */
+int spk_malloc(spk_ctx ctx, size_t bytes, void **out);
+int spk_free(spk_ctx ctx, void *ptr);
+int spk_memcpy_h2d(spk_ctx ctx, void *dst, const void *src, size_t bytes); /* synchronizes */
+int spk_memcpy_d2h(spk_ctx ctx, void *dst, const void *src, size_t bytes); /* synchronizes */
+
+/* ---- 3D real<->complex FFT ----
+ * Replaces init_plan / finish_cufft / r2c_fft / c2r_fft
+ * (src/cuda/cufft_routines.hpp:10-17, cufft_routines.cu:14-101).
+ * Power-of-two sizes, 2 <= NX,NY <= 2048, 2 <= NZ <= 1024.
+ * batch transforms fields at a stride of NTC complex (2*NTC reals). */
+typedef struct spk_fft_plan_s *spk_fft_plan;
+int spk_fft_plan_create(spk_ctx ctx, int nx, int ny, int nz, int precision, int batch,
+                        spk_fft_plan *out);
+int spk_fft_plan_destroy(spk_fft_plan plan);
+/* forward, unnormalized; in: padded real, out: complex; in == out allowed
+ * (replaces r2c_fft, cufft_routines.cu:14-20) */
+int spk_fft_r2c(spk_fft_plan plan, const void *in, void *out);
+/* inverse with the 1/(NX NY NZ) scaling folded in; in == out allowed; when
+ * in != out the input is left untouched (replaces scaleKernel + c2r_fft,
+ * cufft_routines.cu:23-40).  Imaginary parts on the kz=0 and kz=NZ/2 planes
+ * are ignored; the two padding reals of every row are written as 0. */
+int spk_fft_c2r(spk_fft_plan plan, const void *in, void *out);
+
+/* ---- run parameters (Parameters, parameters.cu:77-330) ---- */
+typedef struct spk_params {
+    int nx, ny, nz;
+    int incompressible, heat_equation, boussinesq, stratification, strat_direction;
+    int mhd, anisotropic_diffusion, supertimestepping, shearing, rotating;
+    int with2d, antialiasing;
+    int sts_mask;   /* bit f set: field f is an sts_variable, evolved by RKL2 */
+    int safety_sts;
+    double lx, ly, lz;
+    double nu, nu_m, nu_th, reynolds_ani, N2, OmegaT2, omega, shear;
+    double cfl, cfl_par, cfl_sts, safety_source;
+    double t_initial, t_final, toutput_flow;
+} spk_params;
+
+size_t spk_params_size(void);
+
+/* ---- solver: device state of Fields + TimeStepping + RKLegendre ---- */
+typedef struct spk_solver_s *spk_solver;
+int spk_solver_create(spk_ctx ctx, const spk_params *p, int bulk_precision, spk_solver *out);
+int spk_solver_destroy(spk_solver s);
+/* vars[10] = VX,VY,VZ,BX,BY,BZ,TH,VEL,MAG,NUM_FIELDS; *ntc = NTC */
+int spk_solver_info(spk_solver s, int *vars, size_t *ntc);
+/* device pointers: state (Fields::d_all_fields), dU (d_all_dfields), real buffer
+ * (d_all_buffer_r, padded real [nf][2 NTC]) */
+int spk_solver_buffers(spk_solver s, void **fields, void **dfields, void **realbuf);
+/* time state out8 = t, dt, tremap, dt_hyp, dt_par, dt_sts, t_lastsnap, step (synchronizes) */
+int spk_solver_get_time(spk_solver s, double *out8);
+int spk_solver_set_time(spk_solver s, double t, int step, double t_lastsnap);
+
+/* Fields::CleanFieldDivergence (src/fields/symmetries.cu:114-130) */
+int spk_clean_divergence(spk_solver s);
+/* Supervisor::Complex2RealFields (src/supervisor/supervisor.cu:261-272): state -> real buffer */
+int spk_complex2real(spk_solver s);
+/* TimeStepping::compute_dt (src/timestepping/compute_dt.cu:15-309); dt stays on the device */
+int spk_compute_dt(spk_solver s);
+/* TimeStepping::compute_dfield (src/timestepping/compute_dfield.cu:11-90) into dU */
+int spk_compute_dfield(spk_solver s);
+/* TimeStepping::RungeKutta3 (src/timestepping/hydro_mhd_advance.cu:82-189); uses the current dt */
+int spk_rk3(spk_solver s);
+/* RKLegendre::compute_cycle_RKL2 (src/rkl/rkl.cu:437-591); synchronizes to size the cycle */
+int spk_rkl2_cycle(spk_solver s);
+/* TimeStepping::HydroMHDAdvance (src/timestepping/hydro_mhd_advance.cu:18-75) */
+int spk_hydro_mhd_advance(spk_solver s);
+/* nsteps iterations of Supervisor::executeMainLoop's body without file output
+ * (supervisor.cu:169-190): CleanFieldDivergence + HydroMHDAdvance, with the
+ * snapshot clock of CheckSnapshot (check_output.cu:62-94) kept on the device */
+int spk_main_loop(spk_solver s, int nsteps);
+
+/* ---- measurement (src/io/spooky_output_functions.cu); results synchronize ---- */
+int spk_energy(spk_solver s, int field, double *out);            /* computeEnergy :23-55 */
+int spk_enstrophy(spk_solver s, int vec_first_field, double *out); /* computeEnstrophy :57-86 */
+int spk_correlation(spk_solver s, int fa, int fb, double *out);  /* twoFieldCorrelation :188-235 (real buffer) */
+/* Spectrum1d (cuda_kernels_generic.cu:441-471) + computeSpectrum1d
+ * (spooky_output_spectrum.cu:424-444); out[nbins]; bins beyond nbins are dropped */
+int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPK_H */

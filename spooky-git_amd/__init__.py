@@ -1,0 +1,339 @@
+"""Python binding of libspk (the gfx950 SPOOKY solver) over its C ABI, include/spk.h.
+
+Mirrors the reference's host objects for the hot path:
+  FftPlan  -- init_plan / r2c_fft / c2r_fft / finish_cufft (cufft_routines.hpp:10-17)
+  Solver   -- Fields + TimeStepping + RKLegendre device state and the calls
+              CleanFieldDivergence, compute_dt, compute_dfield, RungeKutta3,
+              compute_cycle_RKL2, HydroMHDAdvance and the main-loop body
+              (symmetries.cu:114, compute_dt.cu:15, compute_dfield.cu:11,
+              hydro_mhd_advance.cu:18-189, rkl.cu:437, supervisor.cu:169)
+The library must be built (spooky-git_amd/build.py); nothing here falls back
+to a CPU path: a missing library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libspk.so")
+
+SPK_F64, SPK_F32 = 0, 1
+_PREC = {"f64": SPK_F64, "fp64": SPK_F64, SPK_F64: SPK_F64, "f32": SPK_F32, "fp32": SPK_F32, SPK_F32: SPK_F32}
+
+PARAM_INT_FIELDS = [
+    "nx", "ny", "nz", "incompressible", "heat_equation", "boussinesq", "stratification",
+    "strat_direction", "mhd", "anisotropic_diffusion", "supertimestepping", "shearing",
+    "rotating", "with2d", "antialiasing", "sts_mask", "safety_sts",
+]
+PARAM_DBL_FIELDS = [
+    "lx", "ly", "lz", "nu", "nu_m", "nu_th", "reynolds_ani", "N2", "OmegaT2", "omega",
+    "shear", "cfl", "cfl_par", "cfl_sts", "safety_source", "t_initial", "t_final",
+    "toutput_flow",
+]
+
+# Parameters defaults (parameters.cu:77-330)
+DEFAULTS = dict(
+    nx=32, ny=32, nz=32, incompressible=0, heat_equation=0, boussinesq=0, stratification=0,
+    strat_direction=2, mhd=0, anisotropic_diffusion=0, supertimestepping=0, shearing=0,
+    rotating=0, with2d=0, antialiasing=1, sts_mask=0, safety_sts=1,
+    lx=1.0, ly=1.0, lz=1.0, nu=1.0, nu_m=1.0, nu_th=1.0, reynolds_ani=1.0, N2=0.0,
+    OmegaT2=0.0, omega=0.0, shear=0.0, cfl=1.5, cfl_par=1.0, cfl_sts=1.0,
+    safety_source=0.2, t_initial=0.0, t_final=1.0, toutput_flow=1.0,
+)
+
+
+class SpkParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in PARAM_INT_FIELDS] + [(n, ctypes.c_double) for n in PARAM_DBL_FIELDS]
+
+
+def make_params(**kw):
+    d = dict(DEFAULTS)
+    for k in kw:
+        if k not in d:
+            raise KeyError("unknown parameter " + k)
+    d.update(kw)
+    p = SpkParams()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+class SpkError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# (name, argtypes) of every exported entry point; restype is int unless noted
+_VP, _I, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+SIGNATURES = {
+    "spk_abi_version": [],
+    "spk_last_error": [],
+    "spk_params_size": [],
+    "spk_ctx_create": [_I, _VP, _VP],
+    "spk_ctx_destroy": [_VP],
+    "spk_ctx_sync": [_VP],
+    "spk_malloc": [_VP, _SZ, _VP],
+    "spk_free": [_VP, _VP],
+    "spk_memcpy_h2d": [_VP, _VP, _VP, _SZ],
+    "spk_memcpy_d2h": [_VP, _VP, _VP, _SZ],
+    "spk_fft_plan_create": [_VP, _I, _I, _I, _I, _I, _VP],
+    "spk_fft_plan_destroy": [_VP],
+    "spk_fft_r2c": [_VP, _VP, _VP],
+    "spk_fft_c2r": [_VP, _VP, _VP],
+    "spk_solver_create": [_VP, _VP, _I, _VP],
+    "spk_solver_destroy": [_VP],
+    "spk_solver_info": [_VP, _VP, _VP],
+    "spk_solver_buffers": [_VP, _VP, _VP, _VP],
+    "spk_solver_get_time": [_VP, _VP],
+    "spk_solver_set_time": [_VP, _D, _I, _D],
+    "spk_clean_divergence": [_VP],
+    "spk_complex2real": [_VP],
+    "spk_compute_dt": [_VP],
+    "spk_compute_dfield": [_VP],
+    "spk_rk3": [_VP],
+    "spk_rkl2_cycle": [_VP],
+    "spk_hydro_mhd_advance": [_VP],
+    "spk_main_loop": [_VP, _I],
+    "spk_energy": [_VP, _I, _VP],
+    "spk_enstrophy": [_VP, _I, _VP],
+    "spk_correlation": [_VP, _I, _I, _VP],
+    "spk_spectrum": [_VP, _I, _I, _VP, _I, _VP],
+}
+
+
+def lib():
+    """Loads libspk.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SpkError("libspk.so not built: run `python spooky-git_amd/build.py` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            if not hasattr(L, name):  # reported by the ABI test; a call raises AttributeError
+                continue
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.spk_last_error.restype = ctypes.c_char_p
+        L.spk_params_size.restype = ctypes.c_size_t
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise SpkError("spk error %d: %s" % (rc, lib().spk_last_error().decode()))
+
+
+def _out_ptr():
+    return ctypes.c_void_p()
+
+
+class Ctx:
+    """Device + stream (spk_ctx).  stream: an int hipStream_t handle or None."""
+
+    def __init__(self, device=0, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().spk_ctx_create(device, ctypes.c_void_p(stream or 0), ctypes.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(lib().spk_ctx_sync(self.h))
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def close(self):
+        if self.h:
+            lib().spk_ctx_destroy(self.h)
+            self.h = None
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().spk_malloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().spk_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def download(self, dtype, shape):
+        a = np.empty(shape, dtype=dtype)
+        assert a.nbytes <= self.nbytes
+        check(lib().spk_memcpy_d2h(self.ctx.h, a.ctypes.data_as(ctypes.c_void_p), self.ptr, a.nbytes))
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib().spk_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_ptr(ctx, ptr, dtype, shape):
+    a = np.empty(shape, dtype=dtype)
+    check(lib().spk_memcpy_d2h(ctx.h, a.ctypes.data_as(ctypes.c_void_p), ptr, a.nbytes))
+    return a
+
+
+def upload_ptr(ctx, ptr, arr):
+    a = np.ascontiguousarray(arr)
+    check(lib().spk_memcpy_h2d(ctx.h, ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+
+class FftPlan:
+    """3D R2C/C2R plan on the reference's padded in-place layout."""
+
+    def __init__(self, ctx, nx, ny, nz, precision="f64", batch=1):
+        self.ctx = ctx
+        self.shape = (nx, ny, nz)
+        self.prec = _PREC[precision]
+        self.batch = batch
+        h = ctypes.c_void_p()
+        check(lib().spk_fft_plan_create(ctx.h, nx, ny, nz, self.prec, batch, ctypes.byref(h)))
+        self.h = h
+
+    def r2c(self, d_in, d_out):
+        check(lib().spk_fft_r2c(self.h, d_in, d_out))
+
+    def c2r(self, d_in, d_out):
+        check(lib().spk_fft_c2r(self.h, d_in, d_out))
+
+    def destroy(self):
+        if self.h:
+            lib().spk_fft_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+VAR_NAMES = ["VX", "VY", "VZ", "BX", "BY", "BZ", "TH", "VEL", "MAG", "NUM_FIELDS"]
+
+
+class Solver:
+    """Device state and time stepping of one run (Fields + TimeStepping + RKLegendre)."""
+
+    def __init__(self, ctx, precision="f64", **params):
+        self.ctx = ctx
+        self.params = make_params(**params)
+        self.prec = _PREC[precision]
+        h = ctypes.c_void_p()
+        check(lib().spk_solver_create(ctx.h, ctypes.byref(self.params), self.prec, ctypes.byref(h)))
+        self.h = h
+        v = (ctypes.c_int * 10)()
+        ntc = ctypes.c_size_t()
+        check(lib().spk_solver_info(h, v, ctypes.byref(ntc)))
+        self.vars = dict(zip(VAR_NAMES, list(v)))
+        self.nf = self.vars["NUM_FIELDS"]
+        self.ntc = ntc.value
+        p = self.params
+        self.shape_c = (p.nx, p.ny, p.nz // 2 + 1)
+        f, d, r = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().spk_solver_buffers(h, ctypes.byref(f), ctypes.byref(d), ctypes.byref(r)))
+        self.d_fields, self.d_dfields, self.d_realbuf = f, d, r
+
+    def destroy(self):
+        if self.h:
+            lib().spk_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # state transfer
+    def set_fields(self, c):
+        c = np.ascontiguousarray(c, dtype=np.complex128).reshape((self.nf,) + self.shape_c)
+        upload_ptr(self.ctx, self.d_fields, c)
+
+    def get_fields(self):
+        return download_ptr(self.ctx, self.d_fields, np.complex128, (self.nf,) + self.shape_c)
+
+    def get_dfields(self):
+        return download_ptr(self.ctx, self.d_dfields, np.complex128, (self.nf,) + self.shape_c)
+
+    def get_realbuf(self):
+        p = self.params
+        a = download_ptr(self.ctx, self.d_realbuf, np.float64, (self.nf, p.nx, p.ny, 2 * (p.nz // 2 + 1)))
+        return a[..., : p.nz]
+
+    def time(self):
+        o = (ctypes.c_double * 8)()
+        check(lib().spk_solver_get_time(self.h, o))
+        return dict(zip(["t", "dt", "tremap", "dt_hyp", "dt_par", "dt_sts", "t_lastsnap", "step"], list(o)))
+
+    def set_time(self, t, step=0, t_lastsnap=0.0):
+        check(lib().spk_solver_set_time(self.h, t, step, t_lastsnap))
+
+    # hot path
+    def clean_divergence(self):
+        check(lib().spk_clean_divergence(self.h))
+
+    def complex2real(self):
+        check(lib().spk_complex2real(self.h))
+
+    def compute_dt(self):
+        check(lib().spk_compute_dt(self.h))
+        return self.time()["dt"]
+
+    def compute_dfield(self):
+        check(lib().spk_compute_dfield(self.h))
+
+    def rk3(self):
+        check(lib().spk_rk3(self.h))
+
+    def rkl2_cycle(self):
+        check(lib().spk_rkl2_cycle(self.h))
+
+    def advance(self):
+        check(lib().spk_hydro_mhd_advance(self.h))
+
+    def main_loop(self, nsteps):
+        check(lib().spk_main_loop(self.h, nsteps))
+
+    # measurement
+    def energy(self, f):
+        o = ctypes.c_double()
+        check(lib().spk_energy(self.h, f, ctypes.byref(o)))
+        return o.value
+
+    def enstrophy(self, f0):
+        o = ctypes.c_double()
+        check(lib().spk_enstrophy(self.h, f0, ctypes.byref(o)))
+        return o.value
+
+    def correlation(self, fa, fb):
+        o = ctypes.c_double()
+        check(lib().spk_correlation(self.h, fa, fb, ctypes.byref(o)))
+        return o.value
+
+    def spectrum(self, fa, fb, cap=4096):
+        out = np.zeros(cap)
+        nb = ctypes.c_int()
+        check(lib().spk_spectrum(self.h, fa, fb, out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(nb)))
+        return out[: nb.value].copy()
+
+
+def padded_real(r):
+    nx, ny, nz = r.shape[-3:]
+    out = np.zeros(r.shape[:-1] + (2 * (nz // 2 + 1),))
+    out[..., :nz] = r
+    return out

@@ -1,0 +1,242 @@
+// spk_device.hpp -- device building blocks for the gfx950 pseudospectral solver.
+//
+// Complex type, constant twiddles, in-register DFTs of radix 2..16, and the
+// LDS-exchanged Stockham line FFT used by every transform pass.
+//
+// Line FFT layout ("natural layout"): a line of N points is held by T = N/E
+// threads, thread t owning points t + i*T for i = 0..E-1 in registers v[i].
+// Radix-E Stockham passes (Govindaraju et al., SC08, "High performance
+// discrete Fourier transforms on graphics processors") run with one LDS
+// exchange between passes; a final radix-b pass (b = N / E^a < E) needs no
+// exchange because its butterflies fall on each thread's own registers.
+// Input and output are both in natural layout, so the first pass reads global
+// memory directly and the last pass writes it directly, coalesced across t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spk {
+
+template <class T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <class T>
+__device__ __forceinline__ cpx<T> mk(T a, T b) { return cpx<T>{a, b}; }
+template <class T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <class T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+template <class TO, class TI>
+__device__ __forceinline__ cpx<TO> cconv(cpx<TI> a) { return {(TO)a.x, (TO)a.y}; }
+
+// cos(2*pi*p/32), p = 0..8, to 20 digits
+__host__ __device__ constexpr double cos32(int p) {
+    return p == 0 ? 1.0
+         : p == 1 ? 0.98078528040323044913
+         : p == 2 ? 0.92387953251128675613
+         : p == 3 ? 0.83146961230254523708
+         : p == 4 ? 0.70710678118654752440
+         : p == 5 ? 0.55557023301960222474
+         : p == 6 ? 0.38268343236508977173
+         : p == 7 ? 0.19509032201612826785
+                  : 0.0;
+}
+// cos and sin of 2*pi*p/32 for any p via octant symmetry
+__host__ __device__ constexpr double ccos32(int p) {
+    p = ((p % 32) + 32) % 32;
+    return p <= 8 ? cos32(p) : p <= 16 ? -cos32(16 - p) : p <= 24 ? -cos32(p - 16) : cos32(32 - p);
+}
+__host__ __device__ constexpr double csin32(int p) { return ccos32(8 - p); }
+
+// multiply by exp(DIR * 2*pi*i * P/Q) with Q | 32, constant-folded
+template <int P, int Q, int DIR, class T>
+__device__ __forceinline__ cpx<T> wconst(cpx<T> a) {
+    static_assert(32 % Q == 0, "Q must divide 32");
+    constexpr int p = (((P * (32 / Q)) % 32) + 32) % 32;  // in units of 2pi/32
+    if constexpr (p == 0) {
+        return a;
+    } else if constexpr (p == 8) {  // exp(DIR i pi/2) = DIR*i
+        return DIR > 0 ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+    } else if constexpr (p == 16) {
+        return cpx<T>{-a.x, -a.y};
+    } else if constexpr (p == 24) {  // exp(DIR i 3pi/2) = -DIR*i
+        return DIR > 0 ? cpx<T>{a.y, -a.x} : cpx<T>{-a.y, a.x};
+    } else {
+        constexpr T c = (T)ccos32(p);
+        constexpr T s = (T)(DIR * csin32(p));
+        return cpx<T>{a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+// ---- in-register DFTs, natural order in and out: X[k] = sum_n x[n] w^{nk}, w = exp(DIR 2 pi i / R)
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft2(cpx<T>& a, cpx<T>& b) {
+    cpx<T> t = a;
+    a = t + b;
+    b = t - b;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft4(cpx<T>& x0, cpx<T>& x1, cpx<T>& x2, cpx<T>& x3) {
+    cpx<T> a = x0 + x2, b = x0 - x2, c = x1 + x3, d = wconst<1, 4, DIR>(x1 - x3);
+    x0 = a + c;
+    x2 = a - c;
+    x1 = b + d;
+    x3 = b - d;
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft8(cpx<T> (&v)[8]) {
+    // n = n1 + 2 n2 (n1<2, n2<4), k = k2 + 4 k1
+    dft4<DIR>(v[0], v[2], v[4], v[6]);
+    dft4<DIR>(v[1], v[3], v[5], v[7]);
+    v[3] = wconst<1, 8, DIR>(v[3]);
+    v[5] = wconst<2, 8, DIR>(v[5]);
+    v[7] = wconst<3, 8, DIR>(v[7]);
+    cpx<T> o[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) {
+        o[k2] = v[2 * k2] + v[2 * k2 + 1];
+        o[k2 + 4] = v[2 * k2] - v[2 * k2 + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = o[i];
+}
+
+template <int DIR, class T>
+__device__ __forceinline__ void dft16(cpx<T> (&v)[16]) {
+    // n = n1 + 4 n2, k = k2 + 4 k1
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++) dft4<DIR>(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+    // now v[n1 + 4 k2] = Y[n1][k2]; twiddle W16^{n1 k2}
+    v[5] = wconst<1, 16, DIR>(v[5]);
+    v[6] = wconst<2, 16, DIR>(v[6]);
+    v[7] = wconst<3, 16, DIR>(v[7]);
+    v[9] = wconst<2, 16, DIR>(v[9]);
+    v[10] = wconst<4, 16, DIR>(v[10]);
+    v[11] = wconst<6, 16, DIR>(v[11]);
+    v[13] = wconst<3, 16, DIR>(v[13]);
+    v[14] = wconst<6, 16, DIR>(v[14]);
+    v[15] = wconst<9, 16, DIR>(v[15]);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) dft4<DIR>(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+    // v[k1 + 4 k2] = X[k2 + 4 k1]: transpose 4x4
+    cpx<T> o[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++)
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) o[k2 + 4 * k1] = v[k1 + 4 * k2];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = o[i];
+}
+
+// DFT of R values taken from v at indices OFF + r*STRIDE (compile-time), in place
+template <int R, int DIR, int OFF, int STRIDE, int E, class T>
+__device__ __forceinline__ void dft_group(cpx<T> (&v)[E]) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        dft2<DIR>(v[OFF], v[OFF + STRIDE]);
+    } else if constexpr (R == 4) {
+        dft4<DIR>(v[OFF], v[OFF + STRIDE], v[OFF + 2 * STRIDE], v[OFF + 3 * STRIDE]);
+    } else {
+        cpx<T> g[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) g[r] = v[OFF + r * STRIDE];
+        if constexpr (R == 8) dft8<DIR>(g);
+        else dft16<DIR>(g);
+#pragma unroll
+        for (int r = 0; r < R; r++) v[OFF + r * STRIDE] = g[r];
+    }
+}
+
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+__host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// LDS index with one pad slot every 16 elements (breaks power-of-two strides)
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_len(int n) { return n + n / 16 + 1; }
+
+// Twiddle table convention: tw[e] = exp(-2 pi i e / N), e = 0..N-1 (forward).
+template <int DIR, class T>
+__device__ __forceinline__ cpx<T> tw_load(const cpx<T>* __restrict__ tw, int e) {
+    cpx<T> w = tw[e];
+    if constexpr (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+// Stockham line FFT, natural layout in and out.  lds: per-line region of
+// lds_len(N) elements.  Caller guarantees every thread of the block executes
+// this (barriers inside).
+template <class T, int N, int E, int DIR>
+struct LineFFT {
+    static_assert(N % E == 0, "E must divide N");
+    static constexpr int TPL = N / E;                  // threads per line
+    static constexpr int NPASS = (E > 1) ? ilog2c(N) / ilog2c(E) : 0;  // full radix-E passes
+    static constexpr int B = (E > 1) ? N / ipow(E, NPASS) : 1;  // final radix (< E)
+
+    template <int P>
+    __device__ static __forceinline__ void pass_full(cpx<T> (&v)[E], cpx<T>* lds, int t,
+                                                     const cpx<T>* __restrict__ tw) {
+        constexpr int Ns = ipow(E, P);
+        const int jm = t % Ns;
+        if constexpr (P > 0) {
+#pragma unroll
+            for (int r = 1; r < E; r++) v[r] = cmul(v[r], tw_load<DIR>(tw, jm * r * (N / (Ns * E))));
+        }
+        dft_group<E, DIR, 0, 1>(v);
+        constexpr bool last = (P == NPASS - 1) && (B == 1);
+        if constexpr (!last) {
+            const int idxD = (t / Ns) * Ns * E + jm;
+#pragma unroll
+            for (int r = 0; r < E; r++) lds[lpad(idxD + r * Ns)] = v[r];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; i++) v[i] = lds[lpad(t + i * TPL)];
+            __syncthreads();
+        }
+        if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw);
+    }
+
+    template <int M>
+    __device__ static __forceinline__ void pass_tail(cpx<T> (&v)[E], int t, const cpx<T>* __restrict__ tw) {
+        // radix-B butterflies j = t + M*TPL on registers v[M + r*(E/B)]
+        constexpr int G = E / B;
+        const int j = t + M * TPL;
+#pragma unroll
+        for (int r = 1; r < B; r++) v[M + r * G] = cmul(v[M + r * G], tw_load<DIR>(tw, j * r));
+        dft_group<B, DIR, M, G>(v);
+        if constexpr (M + 1 < G) pass_tail<M + 1>(v, t, tw);
+    }
+
+    __device__ static __forceinline__ void run(cpx<T> (&v)[E], cpx<T>* lds, int t,
+                                               const cpx<T>* __restrict__ tw) {
+        if constexpr (E == 1) {
+            return;
+        } else {
+            if constexpr (NPASS > 0) pass_full<0>(v, lds, t, tw);
+            if constexpr (B > 1) pass_tail<0>(v, t, tw);
+        }
+    }
+};
+
+// elements per thread for a line of N points
+__host__ __device__ constexpr int line_e(int n, int emax) { return n < emax ? n : emax; }
+
+}  // namespace spk

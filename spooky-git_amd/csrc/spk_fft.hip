@@ -1,0 +1,385 @@
+// spk_fft.hip -- 3D R2C/C2R FFT for gfx950: one kernel per direction pass.
+//
+// Forward transform = z pass (real rows -> half-complex rows, via a complex FFT
+// of length NZ/2 and the split-radix real post-processing step) -> y pass ->
+// x pass.  Inverse = x pass (with the 1/(NX NY NZ) scale applied to the input,
+// as scaleKernel does in cufft_routines.cu:30) -> y pass -> z pass.
+//
+// x and y passes: a workgroup owns TCOLS adjacent columns (contiguous in
+// memory) and the whole line along the transformed axis, so every global load
+// and store moves TCOLS*sizeof(complex) contiguous bytes per line point.
+// z passes: a workgroup owns RW whole rows.
+#include "spk_internal.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace spk {
+
+// ------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------
+
+__host__ __device__ constexpr int strided_e(int n) { return line_e(n, 16); }
+__host__ __device__ constexpr int strided_tpl(int n) { return n / strided_e(n); }
+__host__ __device__ constexpr int strided_tcols(int n) {
+    return strided_tpl(n) >= 256 ? 1 : 256 / strided_tpl(n);
+}
+template <int N>
+__host__ __device__ constexpr bool line_needs_lds() {
+    constexpr int E = strided_e(N);
+    return !(E == N);
+}
+
+// Line FFT over columns: element (f, o, n, c) at f*fstride + o*ostride + n*lstride + c.
+template <class T, int N, int DIR>
+__global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_pass_strided(
+    const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
+    int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale) {
+    constexpr int E = strided_e(N);
+    constexpr int TPL = N / E;
+    constexpr int TC = strided_tcols(N);
+    constexpr int LL = lds_len(N);
+    __shared__ cpx<T> lds[line_needs_lds<N>() ? TC * LL : 1];
+
+    const int tid = threadIdx.x;
+    const int c = tid % TC, t = tid / TC;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = b % ntiles, o = b / ntiles;
+    const long long col = (long long)tile * TC + c;
+    const bool active = col < ncols;
+    const long long base_in = (long long)blockIdx.y * fs_in + (long long)o * ostride + col;
+    const long long base_out = (long long)blockIdx.y * fs_out + (long long)o * ostride + col;
+
+    cpx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        cpx<T> a = active ? in[base_in + (long long)(t + i * TPL) * lstride] : cpx<T>{0, 0};
+        v[i] = cscale(a, scale);
+    }
+    LineFFT<T, N, E, DIR>::run(v, lds + c * LL, t, tw);
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < E; i++) out[base_out + (long long)(t + i * TPL) * lstride] = v[i];
+    }
+}
+
+__host__ __device__ constexpr int zpass_e(int m) { return line_e(m, 16); }
+__host__ __device__ constexpr int zpass_tpl(int m) { return m / zpass_e(m); }
+__host__ __device__ constexpr int zpass_rw(int m) { return zpass_tpl(m) >= 256 ? 1 : 256 / zpass_tpl(m); }
+
+// Real rows -> half-complex rows.  Row r of field f: real input at
+// in + f*fs_in*2 + r*2*nzh (reals), complex output at out + f*fs_out + r*nzh.
+template <class T, int NZ>
+__global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z_r2c(
+    const T *in, cpx<T> *out, long long nrows, long long fs_in, long long fs_out,
+    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m) {
+    constexpr int M = NZ / 2;
+    constexpr int NZH = M + 1;
+    constexpr int E = zpass_e(M);
+    constexpr int TPL = M / E;
+    constexpr int RW = zpass_rw(M);
+    constexpr int LL = lds_len(M + 1);
+    __shared__ cpx<T> lds[RW * LL];
+
+    const int tid = threadIdx.x;
+    const int rl = tid / TPL, t = tid % TPL;
+    const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * RW + rl;
+    const bool active = row < nrows;
+    const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in + (long long)blockIdx.y * fs_in * 2) + row * NZH;
+    cpx<T> *dst = out + (long long)blockIdx.y * fs_out + row * NZH;
+    cpx<T> *L = lds + rl * LL;
+
+    cpx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = active ? src[t + i * TPL] : cpx<T>{0, 0};
+    LineFFT<T, M, E, -1>::run(v, L, t, tw_m);
+#pragma unroll
+    for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
+    __syncthreads();
+    if (active) {
+        const T h = (T)0.5;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int k = t + i * TPL;
+            const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
+            const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
+            const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
+            dst[k] = ev + cmul(od, tw_nz[k]);
+        }
+        if (t == 0) {
+            const cpx<T> z0 = L[0];
+            dst[M] = cpx<T>{z0.x - z0.y, (T)0};
+        }
+    }
+}
+
+// Half-complex rows -> real rows (padding reals written as 0).
+template <class T, int NZ>
+__global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z_c2r(
+    const cpx<T> *in, T *out, long long nrows, long long fs_in, long long fs_out,
+    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m) {
+    constexpr int M = NZ / 2;
+    constexpr int NZH = M + 1;
+    constexpr int E = zpass_e(M);
+    constexpr int TPL = M / E;
+    constexpr int RW = zpass_rw(M);
+    constexpr int LL = lds_len(M + 1);
+    __shared__ cpx<T> lds[RW * LL];
+
+    const int tid = threadIdx.x;
+    const int rl = tid / TPL, t = tid % TPL;
+    const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * RW + rl;
+    const bool active = row < nrows;
+    const cpx<T> *src = in + (long long)blockIdx.y * fs_in + row * NZH;
+    cpx<T> *dstc = reinterpret_cast<cpx<T> *>(out + (long long)blockIdx.y * fs_out * 2) + row * NZH;
+    cpx<T> *L = lds + rl * LL;
+
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int k = t + i * TPL;
+            cpx<T> a = src[k];
+            if (k == 0) a.y = (T)0;
+            L[lpad(k)] = a;
+        }
+        if (t == 0) {
+            cpx<T> a = src[M];
+            a.y = (T)0;
+            L[lpad(M)] = a;
+        }
+    }
+    __syncthreads();
+    cpx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int k = t + i * TPL;
+        const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
+        const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
+        const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
+        const cpx<T> wd = cmulc(d, tw_nz[k]);  // W^{-k} d
+        v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
+    }
+    __syncthreads();
+    LineFFT<T, M, E, +1>::run(v, L, t, tw_m);
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < E; i++) dstc[t + i * TPL] = v[i];
+        if (t == 0) dstc[M] = cpx<T>{(T)0, (T)0};
+    }
+}
+
+// ------------------------------------------------------------------------
+// twiddles
+// ------------------------------------------------------------------------
+
+static void fill_tw(std::vector<double> &h, size_t off, int n) {
+    for (int e = 0; e < n; e++) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
+        h[2 * (off + e)] = (double)cosl(a);
+        h[2 * (off + e) + 1] = (double)sinl(a);
+    }
+}
+
+int twiddles_create(spk_ctx ctx, const Grid3 &g, int prec, Twiddles *tw) {
+    // tables: nx | ny | nz | nz/2 (the half-length complex FFT of the z pass)
+    const int mh = g.nz / 2 > 0 ? g.nz / 2 : 1;
+    size_t n = (size_t)g.nx + g.ny + g.nz + mh;
+    std::vector<double> h(2 * n);
+    tw->off_x = 0;
+    tw->off_y = g.nx;
+    tw->off_z = g.nx + g.ny;
+    fill_tw(h, 0, g.nx);
+    fill_tw(h, g.nx, g.ny);
+    fill_tw(h, g.nx + g.ny, g.nz);
+    fill_tw(h, g.nx + g.ny + g.nz, mh);
+    tw->prec = prec;
+    size_t bytes = n * (prec == SPK_F64 ? 16 : 8);
+    SPK_HIP(hipMalloc(&tw->d, bytes));
+    if (prec == SPK_F64) {
+        SPK_HIP(hipMemcpyAsync(tw->d, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        std::vector<float> f(2 * n);
+        for (size_t i = 0; i < 2 * n; i++) f[i] = (float)h[i];
+        SPK_HIP(hipMemcpyAsync(tw->d, f.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    SPK_HIP(hipStreamSynchronize(ctx->stream));
+    return SPK_OK;
+}
+
+void twiddles_destroy(Twiddles *tw) {
+    if (tw->d) (void)hipFree(tw->d);
+    tw->d = nullptr;
+}
+
+template <class T>
+static const cpx<T> *tw_ptr(const Twiddles &tw, long long off) {
+    return reinterpret_cast<const cpx<T> *>(tw.d) + off;
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+
+#define SPK_POW2_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+
+template <class T, int DIR>
+static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride, long long ostride,
+                          int nouter, const cpx<T> *twp, const PassArgs &a) {
+#define SPK_STRIDED_CASE(NN)                                                                       \
+    case NN: {                                                                                     \
+        constexpr int TC = strided_tcols(NN);                                                      \
+        const int ntiles = (int)((ncols + TC - 1) / TC);                                           \
+        dim3 grid((unsigned)ntiles * nouter, a.nfields);                                          \
+        k_pass_strided<T, NN, DIR><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
+            (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, ntiles, a.fstride_in, \
+            a.fstride_out, twp, (T)a.scale);                                                       \
+        break;                                                                                     \
+    }
+    switch (n) {
+        SPK_POW2_CASES(SPK_STRIDED_CASE)
+        default:
+            set_error("line length %d not supported (power of two 2..2048)", n);
+            return SPK_ERR_ARG;
+    }
+#undef SPK_STRIDED_CASE
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+template <int DIR>
+static int pass_dispatch(spk_ctx ctx, int n, long long ncols, long long lstride, long long ostride,
+                         int nouter, const Twiddles &tw, long long off, int tin, int tout,
+                         const PassArgs &a) {
+    SPK_REQUIRE(tin == tout && tin == tw.prec, SPK_ERR_ARG, "pass precision mismatch");
+    if (tw.prec == SPK_F64)
+        return launch_strided<double, DIR>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<double>(tw, off), a);
+    return launch_strided<float, DIR>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<float>(tw, off), a);
+}
+
+int launch_pass_x(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
+                  const PassArgs &a) {
+    const long long cols = (long long)g.ny * g.nzh;
+    if (dir < 0) return pass_dispatch<-1>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
+    return pass_dispatch<+1>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
+}
+
+int launch_pass_y(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
+                  const PassArgs &a) {
+    const long long plane = (long long)g.ny * g.nzh;
+    if (dir < 0) return pass_dispatch<-1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
+    return pass_dispatch<+1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
+}
+
+#define SPK_Z_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+template <class T>
+static int launch_z(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, bool r2c, const PassArgs &a) {
+    const long long nrows = (long long)g.nx * g.ny;
+    const cpx<T> *tz = tw_ptr<T>(tw, tw.off_z);
+    const cpx<T> *tm = tw_ptr<T>(tw, tw.off_z + g.nz);
+#define SPK_Z_CASE(NN)                                                                         \
+    case NN: {                                                                                 \
+        constexpr int RW = zpass_rw(NN / 2);                                                   \
+        dim3 grid((unsigned)((nrows + RW - 1) / RW), a.nfields);                               \
+        const int bs = RW * zpass_tpl(NN / 2);                                                 \
+        if (r2c)                                                                               \
+            k_pass_z_r2c<T, NN><<<grid, bs, 0, ctx->stream>>>((const T *)a.in, (cpx<T> *)a.out, \
+                                                              nrows, a.fstride_in,             \
+                                                              a.fstride_out, tz, tm);          \
+        else                                                                                   \
+            k_pass_z_c2r<T, NN><<<grid, bs, 0, ctx->stream>>>((const cpx<T> *)a.in, (T *)a.out, \
+                                                              nrows, a.fstride_in,             \
+                                                              a.fstride_out, tz, tm);          \
+        break;                                                                                 \
+    }
+    switch (g.nz) {
+        SPK_Z_CASES(SPK_Z_CASE)
+        default:
+            set_error("NZ=%d not supported (power of two 2..1024)", g.nz);
+            return SPK_ERR_ARG;
+    }
+#undef SPK_Z_CASE
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int launch_pass_z_r2c(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout, const PassArgs &a) {
+    SPK_REQUIRE(tin == tout && tin == tw.prec, SPK_ERR_ARG, "pass precision mismatch");
+    return tw.prec == SPK_F64 ? launch_z<double>(ctx, g, tw, true, a) : launch_z<float>(ctx, g, tw, true, a);
+}
+
+int launch_pass_z_c2r(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout, const PassArgs &a) {
+    SPK_REQUIRE(tin == tout && tin == tw.prec, SPK_ERR_ARG, "pass precision mismatch");
+    return tw.prec == SPK_F64 ? launch_z<double>(ctx, g, tw, false, a) : launch_z<float>(ctx, g, tw, false, a);
+}
+
+}  // namespace spk
+
+// ------------------------------------------------------------------------
+// C ABI: plans
+// ------------------------------------------------------------------------
+
+struct spk_fft_plan_s {
+    spk_ctx ctx;
+    spk::Grid3 g;
+    int prec;
+    int batch;
+    spk::Twiddles tw;
+};
+
+extern "C" int spk_fft_plan_create(spk_ctx ctx, int nx, int ny, int nz, int precision, int batch,
+                                   spk_fft_plan *out) {
+    using namespace spk;
+    SPK_REQUIRE(ctx && out, SPK_ERR_ARG, "spk_fft_plan_create: null argument");
+    SPK_REQUIRE(is_pow2(nx) && is_pow2(ny) && is_pow2(nz) && nx >= 2 && ny >= 2 && nz >= 2 &&
+                    nx <= 2048 && ny <= 2048 && nz <= 1024,
+                SPK_ERR_ARG, "spk_fft_plan_create: unsupported grid %dx%dx%d", nx, ny, nz);
+    SPK_REQUIRE(precision == SPK_F64 || precision == SPK_F32, SPK_ERR_ARG, "bad precision %d", precision);
+    SPK_REQUIRE(batch >= 1 && batch <= 65535, SPK_ERR_ARG, "bad batch %d", batch);
+    SPK_HIP(hipSetDevice(ctx->device));
+    spk_fft_plan p = new spk_fft_plan_s();
+    p->ctx = ctx;
+    p->g = make_grid(nx, ny, nz);
+    p->prec = precision;
+    p->batch = batch;
+    int r = twiddles_create(ctx, p->g, precision, &p->tw);
+    if (r != SPK_OK) {
+        delete p;
+        return r;
+    }
+    *out = p;
+    return SPK_OK;
+}
+
+extern "C" int spk_fft_plan_destroy(spk_fft_plan p) {
+    if (!p) return SPK_OK;
+    spk::twiddles_destroy(&p->tw);
+    delete p;
+    return SPK_OK;
+}
+
+extern "C" int spk_fft_r2c(spk_fft_plan p, const void *in, void *out) {
+    using namespace spk;
+    SPK_REQUIRE(p && in && out, SPK_ERR_ARG, "spk_fft_r2c: null argument");
+    PassArgs a{in, out, p->batch, p->g.ntc, p->g.ntc, 1.0};
+    SPK_TRY(launch_pass_z_r2c(p->ctx, p->g, p->tw, p->prec, p->prec, a));
+    PassArgs b{out, out, p->batch, p->g.ntc, p->g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(p->ctx, p->g, p->tw, -1, p->prec, p->prec, b));
+    SPK_TRY(launch_pass_x(p->ctx, p->g, p->tw, -1, p->prec, p->prec, b));
+    return SPK_OK;
+}
+
+extern "C" int spk_fft_c2r(spk_fft_plan p, const void *in, void *out) {
+    using namespace spk;
+    SPK_REQUIRE(p && in && out, SPK_ERR_ARG, "spk_fft_c2r: null argument");
+    const double scale = 1.0 / ((double)p->g.nx * p->g.ny * p->g.nz);
+    PassArgs a{in, out, p->batch, p->g.ntc, p->g.ntc, scale};
+    SPK_TRY(launch_pass_x(p->ctx, p->g, p->tw, +1, p->prec, p->prec, a));
+    PassArgs b{out, out, p->batch, p->g.ntc, p->g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(p->ctx, p->g, p->tw, +1, p->prec, p->prec, b));
+    SPK_TRY(launch_pass_z_c2r(p->ctx, p->g, p->tw, p->prec, p->prec, b));
+    return SPK_OK;
+}

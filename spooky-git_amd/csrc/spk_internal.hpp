@@ -1,0 +1,110 @@
+// spk_internal.hpp -- host-side internals shared by the libspk translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/spk.h"
+#include "spk_device.hpp"
+
+struct spk_ctx_s {
+    int device;
+    hipStream_t stream;
+};
+
+namespace spk {
+
+// thread-local error text behind spk_last_error()
+void set_error(const char *fmt, ...);
+
+// Evaluates a HIP call; on failure records file:line and the HIP message and
+// returns SPK_ERR_HIP from the enclosing function.
+#define SPK_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            spk::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,                \
+                           hipGetErrorString(e_));                                           \
+            return SPK_ERR_HIP;                                                              \
+        }                                                                                    \
+    } while (0)
+
+// Checks the launch that was just issued.
+#define SPK_LAUNCH_CHECK() SPK_HIP(hipGetLastError())
+
+#define SPK_REQUIRE(cond, code, ...)        \
+    do {                                    \
+        if (!(cond)) {                      \
+            spk::set_error(__VA_ARGS__);    \
+            return (code);                  \
+        }                                   \
+    } while (0)
+
+#define SPK_TRY(expr)              \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != SPK_OK) return r_; \
+    } while (0)
+
+inline bool is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Maps a block id so that the blocks one XCD receives (round-robin dispatch,
+// 8 XCDs) cover a contiguous range of logical tiles: neighbouring tiles share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// ---- transform geometry ----
+
+struct Grid3 {
+    int nx, ny, nz, nzh;
+    long long ntc;  // nx*ny*nzh
+};
+
+inline Grid3 make_grid(int nx, int ny, int nz) {
+    Grid3 g;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.nzh = nz / 2 + 1;
+    g.ntc = (long long)nx * ny * g.nzh;
+    return g;
+}
+
+// Twiddle tables for one grid: exp(-2 pi i e / n) for n = nx, ny, nz, in the
+// bulk precision, contiguous: [nx][ny][nz].
+struct Twiddles {
+    void *d = nullptr;  // device
+    int prec = SPK_F64;
+    long long off_x = 0, off_y = 0, off_z = 0;
+};
+
+int twiddles_create(spk_ctx ctx, const Grid3 &g, int prec, Twiddles *tw);
+void twiddles_destroy(Twiddles *tw);
+
+// ---- pass launchers (spk_fft.hip) ----
+// One field per blockIdx.y; field strides are in complex elements.
+struct PassArgs {
+    const void *in;
+    void *out;
+    int nfields;
+    long long fstride_in, fstride_out;  // complex elements (real rows: 2*NTC reals = NTC complex)
+    double scale;                        // multiplies the input (x inverse pass: 1/N)
+};
+
+// x-direction (stride ny*nzh) C2C pass, dir -1 forward / +1 inverse; in/out
+// precision tin/tout, compute precision = tcomp (must equal the twiddle precision)
+int launch_pass_x(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
+                  const PassArgs &a);
+int launch_pass_y(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
+                  const PassArgs &a);
+// z R2C (real padded rows -> complex rows) and C2R (complex rows -> real padded rows)
+int launch_pass_z_r2c(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout,
+                      const PassArgs &a);
+int launch_pass_z_c2r(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout,
+                      const PassArgs &a);
+
+}  // namespace spk

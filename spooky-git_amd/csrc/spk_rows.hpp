@@ -1,0 +1,87 @@
+// spk_rows.hpp -- real<->half-complex transforms of one z row held by TPL threads.
+//
+// A real row x[0..NZ-1] (NZ = 2M) is handled as the complex sequence
+// z[n] = x[2n] + i x[2n+1], n < M, transformed by a complex FFT of length M.
+// Forward post-processing (k = 0..M, Z[M] == Z[0]):
+//   X[k] = E[k] + W^k O[k],  E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i,
+//   W = exp(-2 pi i / NZ).
+// Inverse pre-processing (unnormalized C2R, imaginary parts of X[0], X[M] dropped):
+//   Z'[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]),  z = IDFT_M(Z').
+// Thread t of a row owns z[t + i*TPL], i < E, in both directions, i.e. the real
+// points 2n and 2n+1 for those n: a C2R followed by an R2C on the same row
+// needs no data movement between them.
+#pragma once
+
+#include "spk_device.hpp"
+
+namespace spk {
+
+template <class T, int M, int E>
+struct ZRow {
+    static constexpr int TPL = M / E;
+    static constexpr int LL = lds_len(M + 1);  // LDS elements per row
+
+    // complex row src[0..M] -> z values v (natural layout).  All threads of the
+    // block call this; inactive rows pass active=false.
+    template <class TI>
+    __device__ static __forceinline__ void c2r(const cpx<TI> *src, bool active, cpx<T> *L, int t,
+                                               const cpx<T> *__restrict__ tw_nz,
+                                               const cpx<T> *__restrict__ tw_m, T scale, cpx<T> (&v)[E]) {
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                const int k = t + i * TPL;
+                cpx<T> a = cscale(cconv<T>(src[k]), scale);
+                if (k == 0) a.y = (T)0;
+                L[lpad(k)] = a;
+            }
+            if (t == 0) {
+                cpx<T> a = cscale(cconv<T>(src[M]), scale);
+                a.y = (T)0;
+                L[lpad(M)] = a;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int k = t + i * TPL;
+            const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
+            const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
+            const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
+            const cpx<T> wd = cmulc(d, tw_nz[k]);  // W^{-k} d
+            v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
+        }
+        __syncthreads();
+        LineFFT<T, M, E, +1>::run(v, L, t, tw_m);
+    }
+
+    // z values v (natural layout) -> complex row dst[0..M].  Ends with a barrier
+    // so the caller may reuse L immediately.
+    template <class TO>
+    __device__ static __forceinline__ void r2c(cpx<T> (&v)[E], bool active, cpx<TO> *dst, cpx<T> *L, int t,
+                                               const cpx<T> *__restrict__ tw_nz,
+                                               const cpx<T> *__restrict__ tw_m) {
+        LineFFT<T, M, E, -1>::run(v, L, t, tw_m);
+#pragma unroll
+        for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
+        __syncthreads();
+        if (active) {
+            const T h = (T)0.5;
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                const int k = t + i * TPL;
+                const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
+                const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
+                const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
+                dst[k] = cconv<TO>(ev + cmul(od, tw_nz[k]));
+            }
+            if (t == 0) {
+                const cpx<T> z0 = L[0];
+                dst[M] = cconv<TO>(cpx<T>{z0.x - z0.y, (T)0});
+            }
+        }
+        __syncthreads();
+    }
+};
+
+}  // namespace spk
