@@ -69,6 +69,13 @@ int spk_fft_r2c(spk_fft_plan plan, const void *in, void *out);
  * are ignored; the two padding reals of every row are written as 0. */
 int spk_fft_c2r(spk_fft_plan plan, const void *in, void *out);
 
+/* One pass of a plan, for timing and for callers that fuse their own passes:
+ * axis 0 = x, 1 = y: complex lines, dir -1 forward / +1 inverse, unscaled;
+ * axis 2 = z: dir -1 real rows -> half-complex rows, +1 the reverse (padding
+ * written as 0).  in == out allowed.  No counterpart call in the reference,
+ * whose cufftXtExec runs all three axes at once (cufft_routines.cu:17,35). */
+int spk_fft_pass(spk_fft_plan plan, int axis, int dir, const void *in, void *out);
+
 /* ---- run parameters (Parameters, parameters.cu:77-330) ---- */
 typedef struct spk_params {
     int nx, ny, nz;

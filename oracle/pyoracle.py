@@ -67,6 +67,7 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         vp = ctypes.c_void_p
         L.orc_params_size.restype = ctypes.c_size_t
+        L.orc_num_threads.restype = ctypes.c_int
         for fn in ("orc_fft_r2c", "orc_fft_c2r"):
             getattr(L, fn).argtypes = [ctypes.c_int] * 3 + [vp, vp]
             getattr(L, fn).restype = None

@@ -25,6 +25,14 @@
 
 size_t orc_params_size(void) { return sizeof(orc_params); }
 
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------------- */
 /* 1D FFT: iterative radix-2, twiddles from long-double sin/cos               */
 /* ------------------------------------------------------------------------- */
@@ -296,7 +304,7 @@ orc_sim *orc_sim_create(const orc_params *p) {
     }
     size_t ssz = 2 * s->ntc * (size_t)(s->nsts ? s->nsts : 1);
     s->dU = zalloc(ssz); s->dU0 = zalloc(ssz); s->Uc0 = zalloc(ssz); s->Uc1 = zalloc(ssz);
-    s->t = p->t_initial;
+    s->t = 0.0; /* timestepping.cu:27 */
     return s;
 }
 

@@ -38,6 +38,7 @@ typedef struct orc_params {
 } orc_params;
 
 size_t orc_params_size(void);
+int orc_num_threads(void); /* OpenMP threads the oracle runs on */
 
 /* ---- transforms (cufft_routines.cu:14-40) ---- */
 /* forward R2C, unnormalized; in: padded real, out: complex; in may equal out */

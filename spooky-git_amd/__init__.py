@@ -82,6 +82,7 @@ SIGNATURES = {
     "spk_fft_plan_destroy": [_VP],
     "spk_fft_r2c": [_VP, _VP, _VP],
     "spk_fft_c2r": [_VP, _VP, _VP],
+    "spk_fft_pass": [_VP, _I, _I, _VP, _VP],
     "spk_solver_create": [_VP, _VP, _I, _VP],
     "spk_solver_destroy": [_VP],
     "spk_solver_info": [_VP, _VP, _VP],
@@ -210,6 +211,9 @@ class FftPlan:
 
     def c2r(self, d_in, d_out):
         check(lib().spk_fft_c2r(self.h, d_in, d_out))
+
+    def one_pass(self, axis, direction, d_in, d_out):
+        check(lib().spk_fft_pass(self.h, axis, direction, d_in, d_out))
 
     def destroy(self):
         if self.h:

@@ -239,4 +239,20 @@ struct LineFFT {
 // elements per thread for a line of N points
 __host__ __device__ constexpr int line_e(int n, int emax) { return n < emax ? n : emax; }
 
+// ---- launch shapes shared by the transform kernels ----
+// x/y passes: E=16 points per thread, TPL threads per line, TCOLS columns per block (256 threads)
+__host__ __device__ constexpr int strided_e(int n) { return line_e(n, 16); }
+__host__ __device__ constexpr int strided_tpl(int n) { return n / strided_e(n); }
+__host__ __device__ constexpr int strided_tcols(int n) {
+    return strided_tpl(n) >= 256 ? 1 : 256 / strided_tpl(n);
+}
+template <int N>
+__host__ __device__ constexpr bool line_needs_lds() {
+    return strided_e(N) != N;
+}
+// z passes: rows of M = NZ/2 complex points, RW rows per block
+__host__ __device__ constexpr int zpass_e(int m) { return line_e(m, 16); }
+__host__ __device__ constexpr int zpass_tpl(int m) { return m / zpass_e(m); }
+__host__ __device__ constexpr int zpass_rw(int m) { return zpass_tpl(m) >= 256 ? 1 : 256 / zpass_tpl(m); }
+
 }  // namespace spk

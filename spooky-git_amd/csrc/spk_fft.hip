@@ -10,6 +10,7 @@
 // and store moves TCOLS*sizeof(complex) contiguous bytes per line point.
 // z passes: a workgroup owns RW whole rows.
 #include "spk_internal.hpp"
+#include "spk_rows.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -22,16 +23,6 @@ namespace spk {
 // kernels
 // ------------------------------------------------------------------------
 
-__host__ __device__ constexpr int strided_e(int n) { return line_e(n, 16); }
-__host__ __device__ constexpr int strided_tpl(int n) { return n / strided_e(n); }
-__host__ __device__ constexpr int strided_tcols(int n) {
-    return strided_tpl(n) >= 256 ? 1 : 256 / strided_tpl(n);
-}
-template <int N>
-__host__ __device__ constexpr bool line_needs_lds() {
-    constexpr int E = strided_e(N);
-    return !(E == N);
-}
 
 // Line FFT over columns: element (f, o, n, c) at f*fstride + o*ostride + n*lstride + c.
 template <class T, int N, int DIR>
@@ -66,9 +57,6 @@ __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_pass_stri
     }
 }
 
-__host__ __device__ constexpr int zpass_e(int m) { return line_e(m, 16); }
-__host__ __device__ constexpr int zpass_tpl(int m) { return m / zpass_e(m); }
-__host__ __device__ constexpr int zpass_rw(int m) { return zpass_tpl(m) >= 256 ? 1 : 256 / zpass_tpl(m); }
 
 // Real rows -> half-complex rows.  Row r of field f: real input at
 // in + f*fs_in*2 + r*2*nzh (reals), complex output at out + f*fs_out + r*nzh.
@@ -79,41 +67,22 @@ __global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;
     constexpr int E = zpass_e(M);
-    constexpr int TPL = M / E;
+    using Row = ZRow<T, M, E>;
     constexpr int RW = zpass_rw(M);
-    constexpr int LL = lds_len(M + 1);
-    __shared__ cpx<T> lds[RW * LL];
+    __shared__ cpx<T> lds[RW * Row::LL];
 
     const int tid = threadIdx.x;
-    const int rl = tid / TPL, t = tid % TPL;
+    const int rl = tid / Row::TPL, t = tid % Row::TPL;
     const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * RW + rl;
     const bool active = row < nrows;
     const cpx<T> *src = reinterpret_cast<const cpx<T> *>(in + (long long)blockIdx.y * fs_in * 2) + row * NZH;
     cpx<T> *dst = out + (long long)blockIdx.y * fs_out + row * NZH;
-    cpx<T> *L = lds + rl * LL;
 
     cpx<T> v[E];
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = active ? src[t + i * TPL] : cpx<T>{0, 0};
-    LineFFT<T, M, E, -1>::run(v, L, t, tw_m);
-#pragma unroll
-    for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
-    __syncthreads();
-    if (active) {
-        const T h = (T)0.5;
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int k = t + i * TPL;
-            const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
-            const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
-            const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
-            dst[k] = ev + cmul(od, tw_nz[k]);
-        }
-        if (t == 0) {
-            const cpx<T> z0 = L[0];
-            dst[M] = cpx<T>{z0.x - z0.y, (T)0};
-        }
-    }
+    for (int i = 0; i < E; i++) v[i] = active ? src[t + i * Row::TPL] : cpx<T>{0, 0};
+    // all loads of the block precede the first barrier inside r2c: in == out is safe
+    Row::r2c(v, active, dst, lds + rl * Row::LL, t, tw_nz, tw_m);
 }
 
 // Half-complex rows -> real rows (padding reals written as 0).
@@ -124,49 +93,23 @@ __global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;
     constexpr int E = zpass_e(M);
-    constexpr int TPL = M / E;
+    using Row = ZRow<T, M, E>;
     constexpr int RW = zpass_rw(M);
-    constexpr int LL = lds_len(M + 1);
-    __shared__ cpx<T> lds[RW * LL];
+    __shared__ cpx<T> lds[RW * Row::LL];
 
     const int tid = threadIdx.x;
-    const int rl = tid / TPL, t = tid % TPL;
+    const int rl = tid / Row::TPL, t = tid % Row::TPL;
     const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * RW + rl;
     const bool active = row < nrows;
     const cpx<T> *src = in + (long long)blockIdx.y * fs_in + row * NZH;
     cpx<T> *dstc = reinterpret_cast<cpx<T> *>(out + (long long)blockIdx.y * fs_out * 2) + row * NZH;
-    cpx<T> *L = lds + rl * LL;
 
-    if (active) {
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int k = t + i * TPL;
-            cpx<T> a = src[k];
-            if (k == 0) a.y = (T)0;
-            L[lpad(k)] = a;
-        }
-        if (t == 0) {
-            cpx<T> a = src[M];
-            a.y = (T)0;
-            L[lpad(M)] = a;
-        }
-    }
-    __syncthreads();
     cpx<T> v[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int k = t + i * TPL;
-        const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
-        const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
-        const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
-        const cpx<T> wd = cmulc(d, tw_nz[k]);  // W^{-k} d
-        v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
-    }
-    __syncthreads();
-    LineFFT<T, M, E, +1>::run(v, L, t, tw_m);
+    // the whole row is staged in LDS before any store: in == out is safe
+    Row::c2r(src, active, lds + rl * Row::LL, t, tw_nz, tw_m, (T)1, v);
     if (active) {
 #pragma unroll
-        for (int i = 0; i < E; i++) dstc[t + i * TPL] = v[i];
+        for (int i = 0; i < E; i++) dstc[t + i * Row::TPL] = v[i];
         if (t == 0) dstc[M] = cpx<T>{(T)0, (T)0};
     }
 }
@@ -382,4 +325,15 @@ extern "C" int spk_fft_c2r(spk_fft_plan p, const void *in, void *out) {
     SPK_TRY(launch_pass_y(p->ctx, p->g, p->tw, +1, p->prec, p->prec, b));
     SPK_TRY(launch_pass_z_c2r(p->ctx, p->g, p->tw, p->prec, p->prec, b));
     return SPK_OK;
+}
+
+extern "C" int spk_fft_pass(spk_fft_plan p, int axis, int dir, const void *in, void *out) {
+    using namespace spk;
+    SPK_REQUIRE(p && in && out, SPK_ERR_ARG, "spk_fft_pass: null argument");
+    SPK_REQUIRE(axis >= 0 && axis <= 2 && (dir == -1 || dir == 1), SPK_ERR_ARG, "spk_fft_pass: axis %d dir %d", axis, dir);
+    PassArgs a{in, out, p->batch, p->g.ntc, p->g.ntc, 1.0};
+    if (axis == 0) return launch_pass_x(p->ctx, p->g, p->tw, dir, p->prec, p->prec, a);
+    if (axis == 1) return launch_pass_y(p->ctx, p->g, p->tw, dir, p->prec, p->prec, a);
+    if (dir < 0) return launch_pass_z_r2c(p->ctx, p->g, p->tw, p->prec, p->prec, a);
+    return launch_pass_z_c2r(p->ctx, p->g, p->tw, p->prec, p->prec, a);
 }

@@ -1,0 +1,434 @@
+// spk_kernels.hpp -- device kernels of the solver's hot path (one RHS evaluation
+// = inverse x pass -> inverse y pass -> fused z kernel -> forward y pass ->
+// forward x pass -> spectral epilogue).
+#pragma once
+
+#include "spk_rows.hpp"
+
+namespace spk {
+
+// Device-resident clock of one run (TimeStepping members current_time,
+// current_dt, tremap, dt_hyp, dt_par, dt_sts; InputOutput::t_lastsnap; step).
+struct TimeState {
+    double t, dt, tremap, dt_hyp, dt_par, dt_sts, t_lastsnap, step;
+};
+
+// Spectral grid geometry: k is computed from the index instead of loaded from
+// kvec/mask arrays (wavevector.cu:50-96 restated per point).
+struct KGeom {
+    int nx, ny, nzh;
+    double kfx, kfy, kfz;    // 2 pi / L_i (kfz = 0 with with2d)
+    double thx, thy, thz;    // dealias thresholds (2/3) * kmax_i before rescaling
+    int antialias;
+    double kxshift;          // tremap * shear: sheared kx = kx0 + kxshift * ky (ShearWavevector)
+};
+
+__device__ __forceinline__ void kvec_at(const KGeom &g, int i, int j, int k, double &kx, double &ky,
+                                        double &kz, double &m) {
+    const int ii = i < g.nx / 2 ? i : i - g.nx;  // fmod(i + NX/2, NX) - NX/2
+    const int jj = j < g.ny / 2 ? j : j - g.ny;
+    const double kx0 = g.kfx * (double)ii;
+    ky = g.kfy * (double)jj;
+    kz = g.kfz * (double)k;
+    kx = kx0 + g.kxshift * ky;
+    m = 1.0;
+    if (g.antialias && (fabs(kx0) > g.thx || fabs(ky) > g.thy || fabs(kz) > g.thz)) m = 0.0;
+}
+
+// ------------------------------------------------------------------------
+// inverse x pass from the FP64 state, with optional i*k_d derivative
+// (Gradient, cuda_kernels_generic.cu:249-260) and the 1/N scaling of c2r_fft
+// ------------------------------------------------------------------------
+struct InvSlot {
+    int src;    // state field index
+    int deriv;  // -1: the field itself; 0,1,2: i*k_{x,y,z} * field
+};
+constexpr int MAX_SLOTS = 12;
+struct InvProg {
+    InvSlot s[MAX_SLOTS];
+};
+
+template <class T, int N>
+__global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_inv_x(
+    const cpx<double> *__restrict__ U, cpx<T> *__restrict__ W, long long fs, InvProg prog, KGeom kg,
+    double scale, const cpx<T> *__restrict__ tw) {
+    constexpr int E = strided_e(N);
+    constexpr int TPL = N / E;
+    constexpr int TC = strided_tcols(N);
+    constexpr int LL = lds_len(N);
+    __shared__ cpx<T> lds[line_needs_lds<N>() ? TC * LL : 1];
+
+    const long long ncols = (long long)kg.ny * kg.nzh;
+    const int tid = threadIdx.x;
+    const int c = tid % TC, t = tid / TC;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const long long col = (long long)tile * TC + c;
+    const bool active = col < ncols;
+    const InvSlot sl = prog.s[blockIdx.y];
+    const int j = active ? (int)(col / kg.nzh) : 0;
+    const int k = active ? (int)(col - (long long)j * kg.nzh) : 0;
+    const cpx<double> *src = U + (long long)sl.src * fs + col;
+
+    cpx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = t + i * TPL;
+        cpx<double> a = active ? src[(long long)n * ncols] : cpx<double>{0, 0};
+        if (sl.deriv >= 0) {
+            double kx, ky, kz, m;
+            kvec_at(kg, n, j, k, kx, ky, kz, m);
+            const double kd = sl.deriv == 0 ? kx : (sl.deriv == 1 ? ky : kz);
+            a = cpx<double>{-(kd * a.y), kd * a.x};
+        }
+        v[i] = cconv<T>(cscale(a, scale));
+    }
+    LineFFT<T, N, E, +1>::run(v, lds + c * LL, t, tw);
+    if (active) {
+        cpx<T> *dst = W + (long long)blockIdx.y * fs + col;
+#pragma unroll
+        for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
+    }
+}
+
+// ------------------------------------------------------------------------
+// fused z kernel: C2R of the input rows, real-space products, R2C of the
+// products.  Real-space fields never reach HBM.
+// ------------------------------------------------------------------------
+enum : int { ZP_U = 1, ZP_B = 2, ZP_TH = 4, ZP_G = 8 };
+
+template <int PROG>
+struct ZProg {
+    static constexpr bool U = PROG & ZP_U, B = PROG & ZP_B, TH = PROG & ZP_TH, G = PROG & ZP_G;
+    // inputs: u(3) | B(3) | theta | grad theta(3)
+    static constexpr int IU = 0, IB = IU + (U ? 3 : 0), ITH = IB + (B ? 3 : 0), IG = ITH + (TH ? 1 : 0);
+    static constexpr int NIN = IG + (G ? 3 : 0);
+    // outputs: Basdevant T(5) | emf(3) | u theta(3) | anisotropic flux(3)
+    static constexpr int OT = 0, OE = OT + (U ? 5 : 0), OQ = OE + ((U && B) ? 3 : 0),
+                         OA = OQ + ((U && TH) ? 3 : 0);
+    static constexpr int NOUT = OA + ((G && B) ? 3 : 0);
+};
+
+struct ZConst {
+    double chi, OmegaT2;  // anisotropic conduction: 1/reynolds_ani, mti_frequency_squared
+    int sd;               // strat_direction
+};
+
+// Product q at one real-space point; a[] holds the NIN input values there.
+// Expressions follow the reference kernels term by term:
+//   TracelessShearMatrix[MHD] (cuda_kernels.cu:39-55, 216-234), MagneticEmf
+//   (cuda_kernels.cu:240-252), EnergyFluxVector (:277-289), ComputeBGradTheta
+//   + ComputeAnisotropicHeatFlux (:360-383).
+template <int PROG, class T>
+__device__ __forceinline__ T zproduct(int q, const T *a, const ZConst &zc) {
+    using Z = ZProg<PROG>;
+    const T two = (T)2.0, three = (T)3.0;
+    if constexpr (Z::U) {
+        const T V0 = a[Z::IU], V1 = a[Z::IU + 1], V2 = a[Z::IU + 2];
+        if (q < Z::OE) {
+            if constexpr (Z::B) {
+                const T B0 = a[Z::IB], B1 = a[Z::IB + 1], B2 = a[Z::IB + 2];
+                switch (q - Z::OT) {
+                    case 0: return (two * V0 * V0 - V1 * V1 - V2 * V2 - two * B0 * B0 + B1 * B1 + B2 * B2) / three;
+                    case 1: return V0 * V1 - B0 * B1;
+                    case 2: return V0 * V2 - B0 * B2;
+                    case 3: return (-V0 * V0 + two * V1 * V1 - V2 * V2 + B0 * B0 - two * B1 * B1 + B2 * B2) / three;
+                    default: return V1 * V2 - B1 * B2;
+                }
+            } else {
+                switch (q - Z::OT) {
+                    case 0: return (two * V0 * V0 - V1 * V1 - V2 * V2) / three;
+                    case 1: return V0 * V1;
+                    case 2: return V0 * V2;
+                    case 3: return (-V0 * V0 + two * V1 * V1 - V2 * V2) / three;
+                    default: return V1 * V2;
+                }
+            }
+        }
+        if constexpr (Z::B) {
+            if (q < Z::OQ) {
+                const T B0 = a[Z::IB], B1 = a[Z::IB + 1], B2 = a[Z::IB + 2];
+                switch (q - Z::OE) {
+                    case 0: return -B1 * V2 + B2 * V1;
+                    case 1: return B0 * V2 - B2 * V0;
+                    default: return -B0 * V1 + B1 * V0;
+                }
+            }
+        }
+        if constexpr (Z::TH) {
+            if (q < Z::OA) {
+                const T th = a[Z::ITH];
+                const int c = q - Z::OQ;
+                return (c == 0 ? V0 : (c == 1 ? V1 : V2)) * th;
+            }
+        }
+    }
+    if constexpr (Z::G && Z::B) {
+        const T B0 = a[Z::IB], B1 = a[Z::IB + 1], B2 = a[Z::IB + 2];
+        const T bg = B0 * a[Z::IG] + B1 * a[Z::IG + 1] + B2 * a[Z::IG + 2];
+        const T Bsq = B0 * B0 + B1 * B1 + B2 * B2;
+        const T bs = zc.sd == 0 ? B0 : (zc.sd == 1 ? B1 : B2);
+        const int c = q - Z::OA;
+        const T Bc = c == 0 ? B0 : (c == 1 ? B1 : B2);
+        return (T)zc.chi * Bc * (bg + (T)zc.OmegaT2 * bs) / Bsq;
+    }
+    return (T)0;
+}
+
+// points per thread in the fused z kernel: all NIN input rows stay in registers
+template <class T>
+__host__ __device__ constexpr int zmid_e(int m) {
+    return line_e(m, sizeof(T) == 8 ? 4 : 8);
+}
+template <class T>
+__host__ __device__ constexpr int zmid_rw(int m) {
+    return (m / zmid_e<T>(m)) >= 256 ? 1 : 256 / (m / zmid_e<T>(m));
+}
+
+template <class T, int NZ, int PROG>
+__global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<T> *__restrict__ P,
+                                              long long nrows, long long fs,
+                                              const cpx<T> *__restrict__ tw_nz,
+                                              const cpx<T> *__restrict__ tw_m, ZConst zc,
+                                              double *__restrict__ pmax) {
+    using Z = ZProg<PROG>;
+    constexpr int M = NZ / 2;
+    constexpr int NZH = M + 1;
+    constexpr int E = zmid_e<T>(M);
+    using Row = ZRow<T, M, E>;
+    constexpr int RW = zmid_rw<T>(M);
+    constexpr int NT = RW * Row::TPL;
+    __shared__ cpx<T> lds[RW * Row::LL];
+    __shared__ double smax[6][(NT + 63) / 64];
+
+    const int tid = threadIdx.x;
+    const int rl = tid / Row::TPL, t = tid % Row::TPL;
+    const long long row = (long long)blockIdx.x * RW + rl;
+    const bool active = row < nrows;
+    cpx<T> *L = lds + rl * Row::LL;
+
+    cpx<T> in[Z::NIN][E];
+#pragma unroll
+    for (int f = 0; f < Z::NIN; f++) Row::c2r(W + f * fs + row * NZH, active, L, t, tw_nz, tw_m, (T)1, in[f]);
+
+    if constexpr (Z::U) {
+        if (pmax) {
+            // max |u_i|, |B_i| over the block's physical points (compute_dt.cu:90-106)
+            double mx[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) mx[c] = 0.0;
+            if (active) {
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        mx[c] = fmax(mx[c], fmax(fabs((double)in[Z::IU + c][i].x), fabs((double)in[Z::IU + c][i].y)));
+                        if constexpr (Z::B)
+                            mx[3 + c] = fmax(mx[3 + c], fmax(fabs((double)in[Z::IB + c][i].x), fabs((double)in[Z::IB + c][i].y)));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                double vmax = mx[c];
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+                if ((tid & 63) == 0) smax[c][tid >> 6] = vmax;
+            }
+            __syncthreads();
+            if (tid < 6) {
+                double vmax = 0.0;
+                for (int w = 0; w < (NT + 63) / 64; w++) vmax = fmax(vmax, smax[tid][w]);
+                pmax[(long long)blockIdx.x * 6 + tid] = vmax;
+            }
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < Z::NOUT; q++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            T ae[Z::NIN], ao[Z::NIN];
+#pragma unroll
+            for (int f = 0; f < Z::NIN; f++) {
+                ae[f] = in[f][i].x;
+                ao[f] = in[f][i].y;
+            }
+            v[i] = cpx<T>{zproduct<PROG>(q, ae, zc), zproduct<PROG>(q, ao, zc)};
+        }
+        Row::r2c(v, active, P + q * fs + row * NZH, L, t, tw_nz, tw_m);
+    }
+}
+
+// ------------------------------------------------------------------------
+// spectral epilogue: one RHS (compute_dfield.cu:11-90) per point, then the
+// RK3 stage update (hydro_mhd_advance.cu:114-164) or a plain dU store
+// ------------------------------------------------------------------------
+
+// Field layouts of Parameters::popVariablesGrid (parameters.cu:568-640)
+enum : int { LY_HEAT = 0, LY_HD = 1, LY_HD_TH = 2, LY_MHD = 3, LY_MHD_TH = 4 };
+template <int LY>
+struct Layout {
+    static constexpr bool INC = LY != LY_HEAT;
+    static constexpr bool MHD = LY == LY_MHD || LY == LY_MHD_TH;
+    static constexpr bool HASTH = LY == LY_HEAT || LY == LY_HD_TH || LY == LY_MHD_TH;
+    static constexpr bool BOUSS = INC && HASTH;
+    static constexpr int VX = 0, BX = 3;
+    static constexpr int TH = LY == LY_HEAT ? 0 : (MHD ? 6 : 3);
+    static constexpr int NF = LY == LY_HEAT ? 1 : 3 + (MHD ? 3 : 0) + (HASTH ? 1 : 0);
+};
+
+struct EpiConst {
+    KGeom kg;
+    long long ntc;
+    int stratification, sd, shearing, rot_active;
+    int aniso_rhs;   // theta gets div of the anisotropic flux products
+    int diff_mask;   // bit f: explicit diffusion of field f in this RHS (not an sts variable)
+    double nu, nu_m, nu_th, N2, shear, omega;
+    int oT, oE, oQ, oA;
+    double gam, xi;  // gammaRK, xiRK of the stage
+};
+
+__device__ __forceinline__ cpx<double> cadd(cpx<double> a, cpx<double> b) { return {a.x + b.x, a.y + b.y}; }
+// -I m (sum): (m * sum.y, -m * sum.x)
+__device__ __forceinline__ cpx<double> mul_minus_im(double m, cpx<double> s) { return {m * s.y, -m * s.x}; }
+// I m (sum): (-m * sum.y, m * sum.x)
+__device__ __forceinline__ cpx<double> mul_plus_im(double m, cpx<double> s) { return {-m * s.y, m * s.x}; }
+// a*X + b*Y + c*Z with real coefficients
+__device__ __forceinline__ cpx<double> kdot3(double a, cpx<double> X, double b, cpx<double> Y, double c, cpx<double> Z) {
+    return {a * X.x + b * Y.x + c * Z.x, a * X.y + b * Y.y + c * Z.y};
+}
+// d += f * u when field q is diffused explicitly in this RHS
+__device__ __forceinline__ cpx<double> diffuse(int mask, int q, cpx<double> d, cpx<double> u, double f) {
+    const double g = (mask >> q & 1) ? f : 0.0;
+    return (mask >> q & 1) ? cpx<double>{d.x + g * u.x, d.y + g * u.y} : d;
+}
+
+template <class T, int LY, int MODE>
+__global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, cpx<double> *__restrict__ U,
+                                                  cpx<double> *__restrict__ S, cpx<double> *__restrict__ dU,
+                                                  EpiConst c, const TimeState *__restrict__ ts) {
+    using L = Layout<LY>;
+    constexpr int NF = L::NF, VX = L::VX, BX = L::BX, TH = L::TH;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= c.ntc) return;
+    const long long N = c.ntc;
+    const long long rowi = idx / c.kg.nzh;
+    const int k = (int)(idx - rowi * c.kg.nzh);
+    const int j = (int)(rowi % c.kg.ny);
+    const int i = (int)(rowi / c.kg.ny);
+    double kx, ky, kz, m;
+    kvec_at(c.kg, i, j, k, kx, ky, kz, m);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+
+    cpx<double> d[NF], u[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        d[f] = cpx<double>{0.0, 0.0};
+        u[f] = U[f * N + idx];
+    }
+    auto prod = [&](int q) { return cconv<double>(P[(long long)q * N + idx]); };
+
+    // HyperbolicTerms (physics.cu:30-103)
+    if constexpr (L::INC) {
+        // NonLinHydroAdv (cuda_kernels.cu:82-98), T_zz = -T_xx - T_yy
+        const cpx<double> Txx = prod(c.oT), Txy = prod(c.oT + 1), Txz = prod(c.oT + 2), Tyy = prod(c.oT + 3),
+                          Tyz = prod(c.oT + 4);
+        d[VX] = mul_minus_im(m, kdot3(kx, Txx, ky, Txy, kz, Txz));
+        d[VX + 1] = mul_minus_im(m, kdot3(kx, Txy, ky, Tyy, kz, Tyz));
+        const cpx<double> sxy = {Txx.x + Tyy.x, Txx.y + Tyy.y};
+        d[VX + 2] = mul_minus_im(m, kdot3(kx, Txz, ky, Tyz, -kz, sxy));
+    }
+    if constexpr (L::MHD) {
+        // MagneticShear (cuda_kernels.cu:257-272)
+        const cpx<double> E0 = prod(c.oE), E1 = prod(c.oE + 1), E2 = prod(c.oE + 2);
+        d[BX] = mul_plus_im(m, cpx<double>{ky * E2.x - kz * E1.x, ky * E2.y - kz * E1.y});
+        d[BX + 1] = mul_plus_im(m, cpx<double>{kz * E0.x - kx * E2.x, kz * E0.y - kx * E2.y});
+        d[BX + 2] = mul_plus_im(m, cpx<double>{kx * E1.x - ky * E0.x, kx * E1.y - ky * E0.y});
+    }
+    if constexpr (L::BOUSS) {
+        // NonLinBoussinesqAdv (cuda_kernels.cu:293-304)
+        d[TH] = mul_minus_im(m, kdot3(kx, prod(c.oQ), ky, prod(c.oQ + 1), kz, prod(c.oQ + 2)));
+    }
+    // SourceTerms (physics.cu:105-142)
+    if constexpr (L::BOUSS) {
+        if (c.stratification) {
+            // BoussinesqStrat (cuda_kernels.cu:310-325)
+            // one-hot blend keeps u[] in registers (no select between addresses)
+            const double w0 = c.sd == 0 ? 1.0 : 0.0, w1 = c.sd == 1 ? 1.0 : 0.0, w2 = c.sd == 2 ? 1.0 : 0.0;
+            const cpx<double> us = {w0 * u[VX].x + w1 * u[VX + 1].x + w2 * u[VX + 2].x,
+                                    w0 * u[VX].y + w1 * u[VX + 1].y + w2 * u[VX + 2].y};
+            const cpx<double> mth = {-u[TH].x, -u[TH].y};
+            if (c.sd == 0) d[VX] = cadd(d[VX], mth);
+            else if (c.sd == 1) d[VX + 1] = cadd(d[VX + 1], mth);
+            else d[VX + 2] = cadd(d[VX + 2], mth);
+            d[TH] = cadd(d[TH], cscale(us, c.N2));
+        }
+    }
+    if constexpr (L::INC) {
+        if (c.shearing) {
+            // ShearingFlow (cuda_kernels.cu:332-340)
+            d[VX + 1] = cadd(d[VX + 1], cscale(u[VX], c.shear));
+            if constexpr (L::MHD) d[BX + 1] = cadd(d[BX + 1], cscale(u[BX], -c.shear));
+        }
+        if (c.rot_active) {
+            // CoriolisForce (cuda_kernels.cu:344-355)
+            const double w2 = 2.0 * c.omega;
+            d[VX] = cadd(d[VX], cscale(u[VX + 1], w2));
+            d[VX + 1] = cpx<double>{d[VX + 1].x - w2 * u[VX].x, d[VX + 1].y - w2 * u[VX].y};
+        }
+        // ParabolicTerms (physics.cu:145-230), nablaOpScalar (cuda_kernels_generic.cu:198-214)
+        const double f = -c.nu * k2;
+        d[VX] = diffuse(c.diff_mask, VX, d[VX], u[VX], f);
+        d[VX + 1] = diffuse(c.diff_mask, VX + 1, d[VX + 1], u[VX + 1], f);
+        d[VX + 2] = diffuse(c.diff_mask, VX + 2, d[VX + 2], u[VX + 2], f);
+    }
+    if constexpr (L::MHD) {
+        const double f = -c.nu_m * k2;
+        d[BX] = diffuse(c.diff_mask, BX, d[BX], u[BX], f);
+        d[BX + 1] = diffuse(c.diff_mask, BX + 1, d[BX + 1], u[BX + 1], f);
+        d[BX + 2] = diffuse(c.diff_mask, BX + 2, d[BX + 2], u[BX + 2], f);
+    }
+    if constexpr (L::HASTH) {
+        if (c.aniso_rhs) {
+            // DivergenceMask flag 1 (cuda_kernels_generic.cu:315-334)
+            d[TH] = cadd(d[TH], mul_plus_im(m, kdot3(kx, prod(c.oA), ky, prod(c.oA + 1), kz, prod(c.oA + 2))));
+        } else if (c.diff_mask >> TH & 1) {
+            const cpx<double> t = cscale(u[TH], -c.nu_th * k2);
+            d[TH] = (LY == LY_HEAT) ? t : cadd(d[TH], t);
+        }
+    }
+    // GradPseudoPressure[Shearing] (cuda_kernels.cu:128-197)
+    if constexpr (L::INC) {
+        cpx<double> s = kdot3(kx, d[VX], ky, d[VX + 1], kz, d[VX + 2]);
+        if (c.shearing) {
+            const double sk = c.shear * ky;
+            s = cpx<double>{sk * u[VX].x + kx * d[VX].x + ky * d[VX + 1].x + kz * d[VX + 2].x,
+                            sk * u[VX].y + kx * d[VX].y + ky * d[VX + 1].y + kz * d[VX + 2].y};
+        }
+        const cpx<double> div = {-s.y, s.x};
+        const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+        const double ax = kx * ik2, ay = ky * ik2, az = kz * ik2;
+        d[VX] = cadd(d[VX], cpx<double>{-ax * div.y, ax * div.x});
+        d[VX + 1] = cadd(d[VX + 1], cpx<double>{-ay * div.y, ay * div.x});
+        d[VX + 2] = cadd(d[VX + 2], cpx<double>{-az * div.y, az * div.x});
+    }
+
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) dU[f * N + idx] = d[f];
+    } else {
+        const double dt = ts->dt;
+        const double a = c.gam * dt, b = c.xi * dt;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            // axpyComplex (cuda_kernels_generic.cu:137-144) with a = 1
+            const cpx<double> base = (MODE == 1) ? u[f] : S[f * N + idx];
+            const cpx<double> un = {base.x + a * d[f].x, base.y + a * d[f].y};
+            U[f * N + idx] = un;
+            if constexpr (MODE < 3) S[f * N + idx] = cpx<double>{un.x + b * d[f].x, un.y + b * d[f].y};
+        }
+    }
+}
+
+}  // namespace spk

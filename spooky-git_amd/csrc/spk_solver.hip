@@ -1,0 +1,1123 @@
+// spk_solver.hip -- device state and time stepping of one SPOOKY run.
+//
+// One RHS evaluation (TimeStepping::compute_dfield, compute_dfield.cu:11-90):
+//   k_inv_x      inverse x pass of the FP64 state (+ i k theta for conduction), scaled 1/N
+//   pass y (+1)  inverse y pass, in place on W
+//   k_zmid       z C2R of every input row -> real-space products -> z R2C (+ CFL maxima)
+//   pass y (-1)  forward y pass, in place on P
+//   pass x (-1)  forward x pass, in place on P
+//   k_epilogue   i k derivatives, 2/3 mask, sources, diffusion, Leray projection and the
+//                RK3 stage update, one pass over the spectral points
+// W and P hold the bulk precision (FP64, or FP32 in FP32-bulk mode); the state,
+// the RK3 scratch and every RKL2 array stay FP64.
+#include "spk_internal.hpp"
+#include "spk_kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace spk {
+
+// ------------------------------------------------------------------------
+// compute_dt (compute_dt.cu:15-309): maxima reduction + CFL arithmetic on the device
+// ------------------------------------------------------------------------
+struct DtConst {
+    int incompressible, heat_equation, mhd, boussinesq, aniso, rotating, shearing, supertimestepping;
+    int safety_sts;
+    int sts_v[3], sts_b[3], sts_th;
+    double kxmax, kymax, kzmax;
+    double nu, nu_m, nu_th, reynolds_ani, N2, OmegaT2, omega, shear;
+    double cfl, cfl_par, cfl_sts, safety_source, t_final, toutput_flow;
+    int nblocks;
+};
+
+__global__ __launch_bounds__(256) void k_dt(const double *__restrict__ pmax, DtConst c, TimeState *ts) {
+    __shared__ double sm[6][256];
+    const int tid = threadIdx.x;
+    double mx[6] = {0, 0, 0, 0, 0, 0};
+    if (c.incompressible) {
+        for (int b = tid; b < c.nblocks; b += 256)
+#pragma unroll
+            for (int q = 0; q < 6; q++) mx[q] = fmax(mx[q], pmax[(long long)b * 6 + q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) sm[q][tid] = mx[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s)
+#pragma unroll
+            for (int q = 0; q < 6; q++) sm[q][tid] = fmax(sm[q][tid], sm[q][tid + s]);
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    double dt_hyp = 0.0, dt_par = 0.0, dt_sts = 0.0, dt = ts->dt;
+    double gamma_v = 0.0, gamma_th = 0.0, gamma_par = 0.0, gamma_b = 0.0, gamma_tmp = 0.0, gamma_sts = 0.0;
+    const double kxmax = c.kxmax, kymax = c.kymax, kzmax = c.kzmax, tremap = ts->tremap;
+    if (c.heat_equation) {
+        gamma_tmp = ((kxmax) * (kxmax) + kymax * kymax + kzmax * kzmax) * c.nu_th;
+        if (!c.sts_th) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        dt_par = c.cfl * c.cfl_par / gamma_par;
+        dt_sts = c.cfl_par / gamma_sts;
+        dt = dt_par;
+    }
+    if (c.incompressible) {
+        const double maxfx = sm[0][0], maxfy = sm[1][0], maxfz = sm[2][0];
+        const double kxe = kxmax + fabs(tremap) * kymax;
+        gamma_v = (kxe * maxfx + kymax * maxfy + kzmax * maxfz);
+        if (c.rotating) gamma_v += fabs(c.omega) / c.safety_source;
+        if (c.shearing) gamma_v += fabs(c.shear) / c.safety_source;
+        gamma_tmp = kxe * kxe * c.nu;
+        if (!c.sts_v[0]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        gamma_tmp = kymax * kymax * c.nu;
+        if (!c.sts_v[1]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        gamma_tmp = kzmax * kzmax * c.nu;
+        if (!c.sts_v[2]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        if (c.boussinesq) {
+            gamma_th += pow(fabs(c.N2), 0.5) / c.safety_source;
+            if (c.aniso) {
+                gamma_th += pow(fabs(c.OmegaT2), 0.5) / c.safety_source;
+                gamma_tmp = (kxe * kxe + kymax * kymax + kzmax * kzmax) * (1. / c.reynolds_ani);
+            } else {
+                gamma_tmp = (kxe * kxe + kymax * kymax + kzmax * kzmax) * c.nu_th;
+            }
+            if (!c.sts_th) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        }
+        if (c.mhd) {
+            const double maxbx = sm[3][0], maxby = sm[4][0], maxbz = sm[5][0];
+            gamma_b = kxe * maxbx + kymax * maxby + kzmax * maxbz;
+            gamma_tmp = kxe * kxe * c.nu_m;
+            if (!c.sts_b[0]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+            gamma_tmp = kymax * kymax * c.nu_m;
+            if (!c.sts_b[1]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+            gamma_tmp = kzmax * kzmax * c.nu_m;
+            if (!c.sts_b[2]) gamma_par += gamma_tmp; else gamma_sts += gamma_tmp;
+        }
+        dt_hyp = c.cfl / (gamma_v + gamma_th + gamma_b);
+        dt_par = c.cfl_par / gamma_par;
+        dt_sts = c.cfl_sts / gamma_sts;
+        dt = (dt_hyp < dt_par) ? dt_hyp : dt_par;
+        if (c.supertimestepping) {
+            if (dt > dt_sts * c.safety_sts) dt = dt_sts * c.safety_sts;
+        }
+    }
+    const double t = ts->t, tls = ts->t_lastsnap;
+    if (t + dt > c.t_final) {
+        dt = c.t_final - t;
+    } else if (t + dt - tls > c.toutput_flow) {
+        dt = c.toutput_flow - t + tls;
+    }
+    if (c.supertimestepping) {
+        if (dt_sts > dt) dt_sts = dt;
+    }
+    ts->dt = dt;
+    ts->dt_hyp = dt_hyp;
+    ts->dt_par = dt_par;
+    ts->dt_sts = dt_sts;
+}
+
+// clock updates: t += dt (RungeKutta3), step++ (HydroMHDAdvance), snapshot
+// clock of CheckSnapshot (check_output.cu:67-83), tremap from the host
+__global__ void k_clock(TimeState *ts, int add_dt, int add_step, int snap, double toutput_flow,
+                        int set_tremap, double tremap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (add_dt) ts->t += ts->dt;
+    if (add_step) ts->step += 1.0;
+    if (set_tremap) ts->tremap = tremap;
+    if (snap && (ts->t - ts->t_lastsnap + 1e-8) >= toutput_flow) ts->t_lastsnap += toutput_flow;
+}
+
+__global__ void k_set_time(TimeState *ts, double t, double step, double tls, double tremap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ts->t = t;
+    ts->step = step;
+    ts->t_lastsnap = tls;
+    ts->tremap = tremap;
+}
+
+// ------------------------------------------------------------------------
+// CleanDivergence (cuda_kernels_generic.cu:288-313), in place on u and B
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_clean_div(cpx<double> *__restrict__ U, long long ntc, KGeom kg,
+                                                   int f0, int f1) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntc) return;
+    const long long rowi = idx / kg.nzh;
+    const int k = (int)(idx - rowi * kg.nzh);
+    const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+    double kx, ky, kz, m;
+    kvec_at(kg, i, j, k, kx, ky, kz, m);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+    const double kk[3] = {kx, ky, kz};
+    for (int g = 0; g < 2; g++) {
+        const int f = g == 0 ? f0 : f1;
+        if (f < 0) continue;
+        cpx<double> X[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) X[q] = U[(f + q) * ntc + idx];
+        const cpx<double> s = kdot3(kx, X[0], ky, X[1], kz, X[2]);
+        const cpx<double> q0 = {-s.y, s.x};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const cpx<double> ikq = {-kk[q] * q0.y, kk[q] * q0.x};  // I k q0
+            U[(f + q) * ntc + idx] = cpx<double>{X[q].x + ikq.x * ik2, X[q].y + ikq.y * ik2};
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// RKL2 stage (rkl.cu:496-585, rkl2_stage cuda_kernels_generic.cu:160-172) fused
+// with ParabolicTermsSTS (physics.cu:233-330) for every sts variable
+// ------------------------------------------------------------------------
+struct RklConst {
+    KGeom kg;
+    long long ntc;
+    int nsts;
+    int field[8];    // state index of sts variable n
+    int kind[8];     // 0: -nu k^2, 1: -nu_m k^2, 2: -nu_th k^2, 3: div of anisotropic flux
+    double nu, nu_m, nu_th;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_rkl2(const cpx<T> *__restrict__ Q, cpx<double> *__restrict__ U,
+                                              cpx<double> *__restrict__ Uc0, cpx<double> *__restrict__ Uc1,
+                                              cpx<double> *__restrict__ dU0, RklConst c, int stage, double mu,
+                                              double nu, double dtmut, double gdt) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= c.ntc) return;
+    const long long N = c.ntc;
+    const long long rowi = idx / c.kg.nzh;
+    const int k = (int)(idx - rowi * c.kg.nzh);
+    const int j = (int)(rowi % c.kg.ny), i = (int)(rowi / c.kg.ny);
+    double kx, ky, kz, m;
+    kvec_at(c.kg, i, j, k, kx, ky, kz, m);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    for (int n = 0; n < c.nsts; n++) {
+        const long long ui = c.field[n] * N + idx, si = n * N + idx;
+        const cpx<double> uc = U[ui];
+        cpx<double> du;
+        if (c.kind[n] == 3) {
+            const cpx<double> q0 = cconv<double>(Q[idx]), q1 = cconv<double>(Q[N + idx]),
+                              q2 = cconv<double>(Q[2 * N + idx]);
+            du = mul_plus_im(m, kdot3(kx, q0, ky, q1, kz, q2));
+        } else {
+            const double a = c.kind[n] == 0 ? c.nu : (c.kind[n] == 1 ? c.nu_m : c.nu_th);
+            du = cscale(uc, -a * k2);
+        }
+        if (stage == 1) {
+            Uc0[si] = uc;
+            dU0[si] = du;
+            Uc1[si] = uc;
+            U[ui] = cpx<double>{uc.x + dtmut * du.x, uc.y + dtmut * du.y};
+        } else {
+            const cpx<double> u1 = Uc1[si], u0 = Uc0[si], d0 = dU0[si];
+            const double w = 1.0 - mu - nu;
+            const cpx<double> Y = {mu * uc.x + nu * u1.x, mu * uc.y + nu * u1.y};
+            Uc1[si] = uc;
+            U[ui] = cpx<double>{Y.x + w * u0.x + dtmut * du.x + gdt * d0.x, Y.y + w * u0.y + dtmut * du.y + gdt * d0.y};
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// shearing-box remap (RemapComplexVec + MaskVector, cuda_kernels_generic.cu:386-423)
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_remap(const cpx<double> *__restrict__ v, cpx<double> *__restrict__ out,
+                                               long long ntc, int nx, int ny, int nzh) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntc) return;
+    const long long rowi = idx / nzh;
+    const int k = (int)(idx - rowi * nzh);
+    const int j = (int)(rowi % ny), i = (int)(rowi / ny);
+    const int ix = i < nx / 2 ? i : i - nx;
+    const int iy = j < ny / 2 ? j : j - ny;
+    int nxt = ix + iy;
+    if (nxt > -nx / 2 && nxt < nx / 2) {
+        if (nxt < 0) nxt += nx;
+        out[(long long)k + (long long)nzh * j + (long long)nzh * ny * nxt] = v[idx];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mask_copy(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ v,
+                                                   long long ntc, KGeom kg) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntc) return;
+    const long long rowi = idx / kg.nzh;
+    const int k = (int)(idx - rowi * kg.nzh);
+    const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+    double kx, ky, kz, m;
+    kvec_at(kg, i, j, k, kx, ky, kz, m);
+    v[idx] = cscale(in[idx], m);
+}
+
+__global__ __launch_bounds__(256) void k_zero(cpx<double> *__restrict__ v, long long n) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) v[idx] = cpx<double>{0.0, 0.0};
+}
+
+// ------------------------------------------------------------------------
+// measurement reductions (spooky_output_functions.cu): fixed-order two-level sums
+// ------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double *sm) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sm[w];
+    return s;
+}
+
+// kind 0: energy weight sum |v|^2 (computeEnergy); kind 1: enstrophy (Curl + computeEnergy)
+__global__ __launch_bounds__(256) void k_spec_sum(const cpx<double> *__restrict__ U, long long ntc, KGeom kg,
+                                                  int kind, double *__restrict__ part) {
+    __shared__ double sm[4];
+    double acc = 0.0;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const double w = k == 0 ? 0.5 : 1.0;
+        if (kind == 0) {
+            const cpx<double> a = U[idx];
+            acc += w * (a.x * a.x + a.y * a.y);
+        } else {
+            const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+            double kx, ky, kz, m;
+            kvec_at(kg, i, j, k, kx, ky, kz, m);
+            const cpx<double> V0 = U[idx], V1 = U[ntc + idx], V2 = U[2 * ntc + idx];
+            const cpx<double> c0 = {-(ky * V2.y - kz * V1.y), ky * V2.x - kz * V1.x};
+            const cpx<double> c1 = {-(kz * V0.y - kx * V2.y), kz * V0.x - kx * V2.x};
+            const cpx<double> c2 = {-(kx * V1.y - ky * V0.y), kx * V1.x - ky * V0.x};
+            acc += w * (c0.x * c0.x + c0.y * c0.y + c1.x * c1.x + c1.y * c1.y + c2.x * c2.x + c2.y * c2.y);
+        }
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// twoFieldCorrelation on physical points of padded real rows
+__global__ __launch_bounds__(256) void k_real_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                                  long long nrows, int nz, int nzp, double *__restrict__ part) {
+    __shared__ double sm[4];
+    double acc = 0.0;
+    const long long n = nrows * nz;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long r = p / nz;
+        const long long o = r * nzp + (p - r * nz);
+        acc += a[o] * b[o];
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_final_sum(const double *__restrict__ part, int n, double scale,
+                                                    double *__restrict__ out) {
+    __shared__ double sm[1024];
+    const int tid = threadIdx.x;
+    sm[tid] = tid < n ? part[tid] : 0.0;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (tid < s) sm[tid] += sm[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) *out = sm[0] * scale;
+}
+
+// Spectrum1d (cuda_kernels_generic.cu:441-471); LDS histogram, then global atomics
+__global__ __launch_bounds__(256) void k_spectrum(const cpx<double> *__restrict__ v1, const cpx<double> *__restrict__ v2,
+                                                  long long ntc, KGeom kg, double ntotal, double deltak, int nbins,
+                                                  double *__restrict__ out) {
+    extern __shared__ double hist[];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0.0;
+    __syncthreads();
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+        double kx, ky, kz, m;
+        kvec_at(kg, i, j, k, kx, ky, kz, m);
+        const cpx<double> a = cscale(v1[idx], 1.0 / ntotal), b = cscale(v2[idx], 1.0 / ntotal);
+        double pw = a.x * b.x + a.y * b.y;
+        if (k > 0) pw *= 2.0;
+        const double kabs = sqrt(kx * kx + ky * ky + kz * kz);
+        const int bin = (int)(kabs / deltak + 0.5);
+        if (bin < nbins) atomicAdd(&hist[bin], pw);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (hist[b] != 0.0) atomicAdd(&out[b], hist[b]);
+}
+
+__global__ void k_zero_d(double *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+}  // namespace spk
+
+// ==========================================================================
+// host side
+// ==========================================================================
+using namespace spk;
+
+struct spk_solver_s {
+    spk_ctx ctx;
+    spk_params p;
+    Grid3 g;
+    int bulk;
+    int ly;
+    int nf, VX, VY, VZ, BX, BY, BZ, TH, VEL, MAG;
+    int sts_pos[8], sts_index[8], nsts;
+    int aniso_rhs, diff_mask;
+    double kxmax, kymax, kzmax, kmax, deltak;
+    int nbins;
+    KGeom kg0;
+    // RHS program
+    int zprog, nin, nout, oT, oE, oQ, oA;
+    InvProg inv;
+    // RKL2 program (anisotropic conduction of theta)
+    int rkl_aniso;
+    InvProg inv_rkl;
+    RklConst rkl;
+    // device buffers
+    cpx<double> *U, *S, *dU;
+    double *realbuf;
+    void *W, *P, *W2, *P2;
+    cpx<double> *rUc0, *rUc1, *rdU0;
+    TimeState *ts;
+    double *pmax, *part, *res, *spec;
+    int zmid_blocks;
+    Twiddles twb, tw64;
+    // host mirror
+    int step;
+    double tremap;
+    bool loop_started, products_valid;
+};
+
+namespace {
+
+size_t esize(int prec) { return prec == SPK_F64 ? 16 : 8; }
+
+template <class T>
+const cpx<T> *twp(const Twiddles &tw, long long off) { return reinterpret_cast<const cpx<T> *>(tw.d) + off; }
+
+KGeom kgeom(const spk_solver_s *s, double kxshift) {
+    KGeom k = s->kg0;
+    k.kxshift = kxshift;
+    return k;
+}
+
+unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+// ---- dispatchers over the grid size ----
+
+template <class T>
+int launch_inv_x(spk_solver s, const InvProg &prog, int nslots, void *W, double kxshift) {
+    const Grid3 &g = s->g;
+    const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+    const cpx<T> *tw = twp<T>(s->twb, s->twb.off_x);
+    KGeom kg = kgeom(s, kxshift);
+#define SPK_INV_CASE(NN)                                                                          \
+    case NN: {                                                                                    \
+        constexpr int TC = strided_tcols(NN);                                                     \
+        dim3 grid((unsigned)(((long long)g.ny * g.nzh + TC - 1) / TC), nslots);                  \
+        k_inv_x<T, NN><<<grid, TC * strided_tpl(NN), 0, s->ctx->stream>>>(s->U, (cpx<T> *)W,     \
+                                                                           g.ntc, prog, kg, scale, tw); \
+        break;                                                                                    \
+    }
+    switch (g.nx) {
+        SPK_INV_CASE(2) SPK_INV_CASE(4) SPK_INV_CASE(8) SPK_INV_CASE(16) SPK_INV_CASE(32) SPK_INV_CASE(64)
+        SPK_INV_CASE(128) SPK_INV_CASE(256) SPK_INV_CASE(512) SPK_INV_CASE(1024) SPK_INV_CASE(2048)
+        default: set_error("NX=%d unsupported", g.nx); return SPK_ERR_ARG;
+    }
+#undef SPK_INV_CASE
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+template <class T, int NZ>
+int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax, unsigned *nblocks_out) {
+    constexpr int M = NZ / 2;
+    constexpr int RW = zmid_rw<T>(M);
+    constexpr int NT = RW * (M / zmid_e<T>(M));
+    const long long nrows = (long long)s->g.nx * s->g.ny;
+    const unsigned nb = (unsigned)((nrows + RW - 1) / RW);
+    if (nblocks_out) *nblocks_out = nb;
+    const cpx<T> *tz = twp<T>(s->twb, s->twb.off_z), *tm = twp<T>(s->twb, s->twb.off_z + s->g.nz);
+    ZConst zc{1.0 / s->p.reynolds_ani, s->p.OmegaT2, s->p.strat_direction};
+    const cpx<T> *w = (const cpx<T> *)W;
+    cpx<T> *pp = (cpx<T> *)P;
+    hipStream_t st = s->ctx->stream;
+    switch (prog) {
+        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U | ZP_B | ZP_TH:
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            break;
+        case ZP_U | ZP_B | ZP_TH | ZP_G:
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            break;
+        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
+    }
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+template <class T>
+int launch_zmid(spk_solver s, int prog, const void *W, void *P, double *pmax, unsigned *nblocks_out) {
+    switch (s->g.nz) {
+        case 2: return launch_zmid_nz<T, 2>(s, prog, W, P, pmax, nblocks_out);
+        case 4: return launch_zmid_nz<T, 4>(s, prog, W, P, pmax, nblocks_out);
+        case 8: return launch_zmid_nz<T, 8>(s, prog, W, P, pmax, nblocks_out);
+        case 16: return launch_zmid_nz<T, 16>(s, prog, W, P, pmax, nblocks_out);
+        case 32: return launch_zmid_nz<T, 32>(s, prog, W, P, pmax, nblocks_out);
+        case 64: return launch_zmid_nz<T, 64>(s, prog, W, P, pmax, nblocks_out);
+        case 128: return launch_zmid_nz<T, 128>(s, prog, W, P, pmax, nblocks_out);
+        case 256: return launch_zmid_nz<T, 256>(s, prog, W, P, pmax, nblocks_out);
+        case 512: return launch_zmid_nz<T, 512>(s, prog, W, P, pmax, nblocks_out);
+        case 1024: return launch_zmid_nz<T, 1024>(s, prog, W, P, pmax, nblocks_out);
+        default: set_error("NZ=%d unsupported", s->g.nz); return SPK_ERR_ARG;
+    }
+}
+
+template <class T, int MODE>
+void launch_epilogue_mode(spk_solver s, const EpiConst &c) {
+    const unsigned nb = nblk(s->g.ntc);
+    hipStream_t st = s->ctx->stream;
+    const cpx<T> *P = (const cpx<T> *)s->P;
+    switch (s->ly) {
+        case LY_HEAT: k_epilogue<T, LY_HEAT, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+        case LY_HD: k_epilogue<T, LY_HD, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+        case LY_HD_TH: k_epilogue<T, LY_HD_TH, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+        case LY_MHD: k_epilogue<T, LY_MHD, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+        default: k_epilogue<T, LY_MHD_TH, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+    }
+}
+
+// mode 0: store dU; 1, 2, 3: RK3 stage update
+template <class T>
+int launch_epilogue(spk_solver s, const EpiConst &c, int mode) {
+    switch (mode) {
+        case 0: launch_epilogue_mode<T, 0>(s, c); break;
+        case 1: launch_epilogue_mode<T, 1>(s, c); break;
+        case 2: launch_epilogue_mode<T, 2>(s, c); break;
+        default: launch_epilogue_mode<T, 3>(s, c); break;
+    }
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+// ---- pipeline pieces ----
+
+// inverse transforms of the RHS inputs and the fused z kernel: products in P
+int rhs_front(spk_solver s, double kxshift, bool with_max) {
+    if (s->nin == 0) return SPK_OK;
+    const Grid3 &g = s->g;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv, s->nin, s->W, kxshift));
+    else SPK_TRY(launch_inv_x<float>(s, s->inv, s->nin, s->W, kxshift));
+    PassArgs a{s->W, s->W, s->nin, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    double *pm = with_max ? s->pmax : nullptr;
+    unsigned nb = 0;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, s->zprog, s->W, s->P, pm, &nb));
+    else SPK_TRY(launch_zmid<float>(s, s->zprog, s->W, s->P, pm, &nb));
+    s->zmid_blocks = (int)nb;
+    return SPK_OK;
+}
+
+// forward transforms of the products and the spectral epilogue
+int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
+    const Grid3 &g = s->g;
+    if (s->nout > 0) {
+        PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
+        SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+        SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+    }
+    EpiConst c;
+    memset(&c, 0, sizeof(c));
+    c.kg = kgeom(s, kxshift);
+    c.ntc = g.ntc;
+    c.stratification = s->p.stratification;
+    c.sd = s->p.strat_direction;
+    c.shearing = s->p.shearing;
+    c.rot_active = s->p.rotating && s->p.shearing;
+    c.aniso_rhs = s->aniso_rhs;
+    c.diff_mask = s->diff_mask;
+    c.nu = s->p.nu; c.nu_m = s->p.nu_m; c.nu_th = s->p.nu_th; c.N2 = s->p.N2;
+    c.shear = s->p.shear; c.omega = s->p.omega;
+    c.oT = s->oT; c.oE = s->oE; c.oQ = s->oQ; c.oA = s->oA;
+    c.gam = gam;
+    c.xi = xi;
+    if (s->bulk == SPK_F64) return launch_epilogue<double>(s, c, mode);
+    return launch_epilogue<float>(s, c, mode);
+}
+
+int dt_kernel(spk_solver s) {
+    DtConst c;
+    memset(&c, 0, sizeof(c));
+    const spk_params &p = s->p;
+    c.incompressible = p.incompressible && !p.heat_equation;
+    c.heat_equation = p.heat_equation;
+    c.mhd = p.mhd; c.boussinesq = p.boussinesq; c.aniso = p.anisotropic_diffusion;
+    c.rotating = p.rotating; c.shearing = p.shearing; c.supertimestepping = p.supertimestepping;
+    c.safety_sts = p.safety_sts;
+    for (int q = 0; q < 3; q++) {
+        c.sts_v[q] = c.incompressible && s->sts_pos[s->VX + q] >= 0;
+        c.sts_b[q] = p.mhd && s->sts_pos[s->BX + q] >= 0;
+    }
+    c.sts_th = (p.boussinesq || p.heat_equation) && s->sts_pos[s->TH] >= 0;
+    c.kxmax = s->kxmax; c.kymax = s->kymax; c.kzmax = s->kzmax;
+    c.nu = p.nu; c.nu_m = p.nu_m; c.nu_th = p.nu_th; c.reynolds_ani = p.reynolds_ani;
+    c.N2 = p.N2; c.OmegaT2 = p.OmegaT2; c.omega = p.omega; c.shear = p.shear;
+    c.cfl = p.cfl; c.cfl_par = p.cfl_par; c.cfl_sts = p.cfl_sts; c.safety_source = p.safety_source;
+    c.t_final = p.t_final; c.toutput_flow = p.toutput_flow;
+    c.nblocks = s->zmid_blocks;
+    k_dt<<<1, 256, 0, s->ctx->stream>>>(s->pmax, c, s->ts);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int clock(spk_solver s, int add_dt, int add_step, int snap, int set_tremap, double tremap) {
+    k_clock<<<1, 1, 0, s->ctx->stream>>>(s->ts, add_dt, add_step, snap, s->p.toutput_flow, set_tremap, tremap);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int read_time(spk_solver s, TimeState *h) {
+    SPK_HIP(hipMemcpyAsync(h, s->ts, sizeof(TimeState), hipMemcpyDeviceToHost, s->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SPK_OK;
+}
+
+int clean_divergence(spk_solver s) {
+    const int f0 = (s->p.incompressible && !s->p.heat_equation) ? s->VEL : -1;
+    const int f1 = s->p.mhd && f0 >= 0 ? s->MAG : -1;
+    if (f0 < 0 && f1 < 0) return SPK_OK;
+    k_clean_div<<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>(s->U, s->g.ntc, kgeom(s, s->tremap * s->p.shear), f0, f1);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int compute_dt(spk_solver s) {
+    if (s->p.incompressible && !s->p.heat_equation) {
+        SPK_TRY(rhs_front(s, s->tremap * s->p.shear, true));
+        s->products_valid = true;
+    }
+    return dt_kernel(s);
+}
+
+static const double gammaRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double xiRK[2] = {-17.0 / 60.0, -5.0 / 12.0};
+
+int shift_time_host(spk_solver s, double t) {
+    const spk_params &p = s->p;
+    s->tremap = fmod(t + p.ly / (2.0 * p.shear * p.lx), p.ly / (p.shear * p.lx)) - p.ly / (2.0 * p.shear * p.lx);
+    return SPK_OK;
+}
+
+int remap_all_fields(spk_solver s) {
+    const Grid3 &g = s->g;
+    cpx<double> *tmp = reinterpret_cast<cpx<double> *>(s->realbuf);
+    for (int f = 0; f < s->nf; f++) {
+        cpx<double> *v = s->U + (long long)f * g.ntc;
+        k_zero<<<nblk(g.ntc), 256, 0, s->ctx->stream>>>(tmp, g.ntc);
+        k_remap<<<nblk(g.ntc), 256, 0, s->ctx->stream>>>(v, tmp, g.ntc, g.nx, g.ny, g.nzh);
+        k_mask_copy<<<nblk(g.ntc), 256, 0, s->ctx->stream>>>(tmp, v, g.ntc, kgeom(s, 0.0));
+        SPK_LAUNCH_CHECK();
+    }
+    return SPK_OK;
+}
+
+int rk3(spk_solver s) {
+    const spk_params &p = s->p;
+    const bool shear = p.shearing;
+    double dt = 0.0;
+    if (shear) {
+        TimeState h;
+        SPK_TRY(read_time(s, &h));
+        dt = h.dt;
+    }
+    const double ks0 = s->tremap * p.shear;
+    // stage 1
+    if (!s->products_valid) SPK_TRY(rhs_front(s, ks0, false));
+    s->products_valid = false;
+    SPK_TRY(rhs_back(s, ks0, 1, gammaRK[0], xiRK[0]));
+    // stage 2
+    const double ks1 = shear ? (s->tremap + gammaRK[0] * dt) * p.shear : ks0;
+    SPK_TRY(rhs_front(s, ks1, false));
+    SPK_TRY(rhs_back(s, ks1, 2, gammaRK[1], xiRK[1]));
+    // stage 3
+    const double ks2 = shear ? (s->tremap + (gammaRK[0] + xiRK[0] + gammaRK[1]) * dt) * p.shear : ks0;
+    SPK_TRY(rhs_front(s, ks2, false));
+    SPK_TRY(rhs_back(s, ks2, 3, gammaRK[2], 0.0));
+    SPK_TRY(clock(s, 1, 0, 0, 0, 0.0));
+    if (shear) {
+        TimeState h;
+        SPK_TRY(read_time(s, &h));
+        s->tremap += dt;
+        if (s->tremap > p.ly / (2.0 * p.shear * p.lx)) {
+            SPK_TRY(shift_time_host(s, h.t));
+            SPK_TRY(remap_all_fields(s));
+        }
+        SPK_TRY(clock(s, 0, 0, 0, 1, s->tremap));
+    }
+    return SPK_OK;
+}
+
+// RKL2 coefficient recurrence (rkl.cu:466-585); returns the stage count
+int rkl2_coeffs(double dt_current, double dt_sts, std::vector<double> &coef) {
+    const double scrh = dt_current / dt_sts;
+    const double s_str = 4.0 * (1.0 + 2.0 * scrh) / (1.0 + sqrt(9.0 + 16.0 * scrh));
+    const int s_RKL = 1 + (int)(s_str);
+    const double w1 = 4.0 / (s_RKL * s_RKL + s_RKL - 2.0);
+    double mu_tilde_j = w1 / 3.0;
+    double b_j, b_jm1, b_jm2, a_jm1;
+    b_j = b_jm1 = b_jm2 = 1.0 / 3.0;
+    a_jm1 = 1.0 - b_jm1;
+    coef.assign(4 * (size_t)(s_RKL > 0 ? s_RKL : 1), 0.0);
+    coef[2] = mu_tilde_j;
+    for (int stage = 2; stage <= s_RKL; stage++) {
+        const double mu_j = (2. * stage - 1.) / stage * b_j / b_jm1;
+        mu_tilde_j = w1 * mu_j;
+        const double gamma_j = -a_jm1 * mu_tilde_j;
+        const double nu_j = -(stage - 1.) * b_j / (stage * b_jm2);
+        double *c = &coef[4 * (size_t)(stage - 1)];
+        c[0] = mu_j; c[1] = nu_j; c[2] = mu_tilde_j; c[3] = gamma_j;
+        b_jm2 = b_jm1;
+        b_jm1 = b_j;
+        a_jm1 = 1.0 - b_jm1;
+        b_j = 0.5 * (stage * stage + 3.0 * stage) / (stage * stage + 3.0 * stage + 2);
+    }
+    return s_RKL;
+}
+
+template <class T>
+int rkl2_stage_launch(spk_solver s, int stage, double mu, double nu, double dtmut, double gdt) {
+    RklConst c = s->rkl;
+    c.kg = kgeom(s, s->tremap * s->p.shear);
+    k_rkl2<T><<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>((const cpx<T> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c,
+                                                           stage, mu, nu, dtmut, gdt);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int rkl2_front(spk_solver s) {
+    // anisotropic flux of theta from i k theta and B (AnisotropicHeatFlux,
+    // anisotropic_conduction.cu:10-62), into P2
+    const Grid3 &g = s->g;
+    const double ks = s->tremap * s->p.shear;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl, 6, s->W2, ks));
+    else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl, 6, s->W2, ks));
+    PassArgs a{s->W2, s->W2, 6, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, ZP_B | ZP_G, s->W2, s->P2, nullptr, nullptr));
+    else SPK_TRY(launch_zmid<float>(s, ZP_B | ZP_G, s->W2, s->P2, nullptr, nullptr));
+    PassArgs b{s->P2, s->P2, 3, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b));
+    SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b));
+    return SPK_OK;
+}
+
+int rkl2_cycle(spk_solver s) {
+    if (s->nsts == 0) return SPK_OK;
+    TimeState h;
+    SPK_TRY(read_time(s, &h));
+    std::vector<double> coef;
+    const int sr = rkl2_coeffs(h.dt, h.dt_sts, coef);
+    for (int stage = 1; stage <= sr; stage++) {
+        if (s->rkl_aniso) SPK_TRY(rkl2_front(s));
+        const double *c = &coef[4 * (size_t)(stage - 1)];
+        const double mu = c[0], nu = c[1], mut = c[2], gam = c[3];
+        const double dtmut = stage == 1 ? mut * h.dt : h.dt * mut;
+        if (s->bulk == SPK_F64) SPK_TRY(rkl2_stage_launch<double>(s, stage, mu, nu, dtmut, gam * h.dt));
+        else SPK_TRY(rkl2_stage_launch<float>(s, stage, mu, nu, dtmut, gam * h.dt));
+    }
+    return SPK_OK;
+}
+
+int hydro_mhd_advance(spk_solver s) {
+    const bool sts = s->p.supertimestepping && s->nsts > 0;
+    SPK_TRY(compute_dt(s));
+    if (sts && (s->step % 2) == 1) SPK_TRY(rkl2_cycle(s));
+    SPK_TRY(rk3(s));
+    if (sts && (s->step % 2) == 0) SPK_TRY(rkl2_cycle(s));
+    s->step += 1;
+    return clock(s, 0, 1, 0, 0, 0.0);
+}
+
+void free_all(spk_solver s) {
+    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->rUc0, s->rUc1, s->rdU0,
+                    s->ts, s->pmax, s->part, s->res, s->spec};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    twiddles_destroy(&s->twb);
+    twiddles_destroy(&s->tw64);
+}
+
+template <class TPtr>
+int dalloc(TPtr **p, size_t bytes) {
+    hipError_t e = hipMalloc((void **)p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return SPK_ERR_ALLOC;
+    }
+    return SPK_OK;
+}
+
+int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
+    const spk_params &p = *pin;
+    s->ctx = ctx;
+    s->p = p;
+    s->bulk = bulk;
+    SPK_REQUIRE(is_pow2(p.nx) && is_pow2(p.ny) && is_pow2(p.nz) && p.nx >= 2 && p.ny >= 2 && p.nz >= 2 &&
+                    p.nx <= 2048 && p.ny <= 2048 && p.nz <= 1024,
+                SPK_ERR_ARG, "unsupported grid %dx%dx%d (powers of two, NX,NY <= 2048, NZ <= 1024)", p.nx, p.ny, p.nz);
+    SPK_REQUIRE(bulk == SPK_F64 || bulk == SPK_F32, SPK_ERR_ARG, "bad bulk precision %d", bulk);
+    SPK_REQUIRE(p.heat_equation || p.incompressible, SPK_ERR_ARG, "neither incompressible nor heat_equation set");
+    SPK_REQUIRE(!p.anisotropic_diffusion || (p.mhd && p.boussinesq), SPK_ERR_ARG,
+                "anisotropic_diffusion needs mhd and boussinesq (the flux is along B)");
+    SPK_REQUIRE(p.strat_direction >= 0 && p.strat_direction <= 2, SPK_ERR_ARG, "bad strat_direction");
+    SPK_REQUIRE(!p.shearing || p.shear != 0.0, SPK_ERR_ARG, "shearing needs a nonzero shear");
+    s->g = make_grid(p.nx, p.ny, p.nz);
+    // Parameters::popVariablesGrid (parameters.cu:568-640)
+    s->VX = s->VY = s->VZ = s->BX = s->BY = s->BZ = s->TH = s->VEL = s->MAG = 0;
+    if (p.heat_equation) {
+        s->TH = 0; s->nf = 1; s->ly = LY_HEAT;
+    } else {
+        s->VX = 0; s->VY = 1; s->VZ = 2; s->VEL = 0; s->nf = 3; s->ly = LY_HD;
+        if (p.mhd) {
+            s->BX = 3; s->BY = 4; s->BZ = 5; s->MAG = 3; s->nf = 6; s->ly = LY_MHD;
+            if (p.boussinesq) { s->TH = 6; s->nf = 7; s->ly = LY_MHD_TH; }
+        } else if (p.boussinesq) {
+            s->TH = 3; s->nf = 4; s->ly = LY_HD_TH;
+        }
+    }
+    const bool inc = !p.heat_equation;
+    const bool has_th = p.heat_equation || p.boussinesq;
+    // RKLegendre ctor (rkl.cu:40-57): sts variables
+    s->nsts = 0;
+    for (int f = 0; f < 8; f++) s->sts_pos[f] = -1;
+    for (int f = 0; f < s->nf; f++)
+        if (p.supertimestepping && (p.sts_mask >> f & 1)) {
+            s->sts_index[s->nsts] = f;
+            s->sts_pos[f] = s->nsts++;
+        }
+    s->diff_mask = 0;
+    for (int f = 0; f < s->nf; f++)
+        if (s->sts_pos[f] < 0) s->diff_mask |= 1 << f;
+    s->aniso_rhs = has_th && p.anisotropic_diffusion && s->sts_pos[s->TH] < 0;
+    // wavevector.cu:70-110
+    double kxmax = 2.0 * M_PI / p.lx * ((p.nx / 2) - 1);
+    double kymax = 2.0 * M_PI / p.ly * ((p.ny / 2) - 1);
+    double kzmax = 2.0 * M_PI / p.lz * ((p.nz / 2) - 1);
+    if (p.with2d) kzmax = 0.0;
+    KGeom &kg = s->kg0;
+    kg.nx = p.nx; kg.ny = p.ny; kg.nzh = s->g.nzh;
+    kg.kfx = (2.0 * M_PI) / p.lx;
+    kg.kfy = (2.0 * M_PI) / p.ly;
+    kg.kfz = p.with2d ? 0.0 : (2.0 * M_PI) / p.lz;
+    kg.thx = 2.0 / 3.0 * kxmax; kg.thy = 2.0 / 3.0 * kymax; kg.thz = 2.0 / 3.0 * kzmax;
+    kg.antialias = p.antialiasing;
+    kg.kxshift = 0.0;
+    if (p.antialiasing) {
+        kxmax = (2.0 / 3.0) * kxmax;
+        kymax = (2.0 / 3.0) * kymax;
+        kzmax = (2.0 / 3.0) * kzmax;
+    }
+    s->kxmax = kxmax; s->kymax = kymax; s->kzmax = kzmax;
+    s->kmax = pow(kxmax * kxmax + kymax * kymax + kzmax * kzmax, 0.5);
+    double lmin = p.lx;
+    if (p.ly < lmin) lmin = p.ly;
+    if (p.lz < lmin) lmin = p.lz;
+    s->deltak = 2.0 * M_PI / lmin;
+    s->nbins = (int)(s->kmax / s->deltak + 0.5);
+    // RHS program
+    memset(&s->inv, 0, sizeof(s->inv));
+    int ns = 0;
+    s->zprog = 0;
+    if (inc) {
+        for (int q = 0; q < 3; q++) s->inv.s[ns++] = InvSlot{s->VX + q, -1};
+        s->zprog |= ZP_U;
+        if (p.mhd) {
+            for (int q = 0; q < 3; q++) s->inv.s[ns++] = InvSlot{s->BX + q, -1};
+            s->zprog |= ZP_B;
+        }
+        if (p.boussinesq) {
+            s->inv.s[ns++] = InvSlot{s->TH, -1};
+            s->zprog |= ZP_TH;
+        }
+        if (s->aniso_rhs) {
+            for (int q = 0; q < 3; q++) s->inv.s[ns++] = InvSlot{s->TH, q};
+            s->zprog |= ZP_G;
+        }
+    }
+    s->nin = ns;
+    s->oT = 0;
+    s->oE = inc ? 5 : 0;
+    s->oQ = s->oE + (inc && p.mhd ? 3 : 0);
+    s->oA = s->oQ + (inc && p.boussinesq ? 3 : 0);
+    s->nout = inc ? s->oA + (s->aniso_rhs ? 3 : 0) : 0;
+    // RKL2 program
+    s->rkl_aniso = has_th && p.anisotropic_diffusion && s->sts_pos[s->TH] >= 0;
+    memset(&s->inv_rkl, 0, sizeof(s->inv_rkl));
+    for (int q = 0; q < 3; q++) s->inv_rkl.s[q] = InvSlot{s->BX + q, -1};
+    for (int q = 0; q < 3; q++) s->inv_rkl.s[3 + q] = InvSlot{s->TH, q};
+    memset(&s->rkl, 0, sizeof(s->rkl));
+    s->rkl.ntc = s->g.ntc;
+    s->rkl.nsts = s->nsts;
+    s->rkl.nu = p.nu; s->rkl.nu_m = p.nu_m; s->rkl.nu_th = p.nu_th;
+    for (int n = 0; n < s->nsts; n++) {
+        const int f = s->sts_index[n];
+        s->rkl.field[n] = f;
+        if (has_th && f == s->TH) s->rkl.kind[n] = p.anisotropic_diffusion ? 3 : 2;
+        else if (p.mhd && f >= s->BX && f < s->BX + 3 && inc) s->rkl.kind[n] = 1;
+        else s->rkl.kind[n] = 0;
+    }
+    // buffers
+    const long long N = s->g.ntc;
+    const size_t f64 = (size_t)N * 16, fb = (size_t)N * esize(bulk);
+    SPK_TRY(dalloc(&s->U, f64 * s->nf));
+    SPK_TRY(dalloc(&s->S, f64 * s->nf));
+    SPK_TRY(dalloc(&s->dU, f64 * s->nf));
+    SPK_TRY(dalloc(&s->realbuf, f64 * s->nf));
+    SPK_TRY(dalloc(&s->W, fb * (s->nin > 0 ? s->nin : 1)));
+    SPK_TRY(dalloc(&s->P, fb * (s->nout > 0 ? s->nout : 1)));
+    if (s->rkl_aniso) {
+        SPK_TRY(dalloc(&s->W2, fb * 6));
+        SPK_TRY(dalloc(&s->P2, fb * 3));
+    }
+    if (s->nsts > 0) {
+        SPK_TRY(dalloc(&s->rUc0, f64 * s->nsts));
+        SPK_TRY(dalloc(&s->rUc1, f64 * s->nsts));
+        SPK_TRY(dalloc(&s->rdU0, f64 * s->nsts));
+    }
+    SPK_TRY(dalloc(&s->ts, sizeof(TimeState)));
+    const long long rows = (long long)p.nx * p.ny;
+    SPK_TRY(dalloc(&s->pmax, sizeof(double) * 6 * (size_t)(rows + 1)));
+    SPK_TRY(dalloc(&s->part, sizeof(double) * RED_BLOCKS));
+    SPK_TRY(dalloc(&s->res, sizeof(double) * 16));
+    SPK_TRY(dalloc(&s->spec, sizeof(double) * (size_t)(s->nbins + 1)));
+    SPK_TRY(twiddles_create(ctx, s->g, bulk, &s->twb));
+    SPK_TRY(twiddles_create(ctx, s->g, SPK_F64, &s->tw64));
+    SPK_HIP(hipMemsetAsync(s->U, 0, f64 * s->nf, ctx->stream));
+    SPK_HIP(hipMemsetAsync(s->S, 0, f64 * s->nf, ctx->stream));
+    SPK_HIP(hipMemsetAsync(s->dU, 0, f64 * s->nf, ctx->stream));
+    SPK_HIP(hipMemsetAsync(s->realbuf, 0, f64 * s->nf, ctx->stream));
+    TimeState h{};
+    SPK_HIP(hipMemcpyAsync(s->ts, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    SPK_HIP(hipStreamSynchronize(ctx->stream));
+    s->step = 0;
+    s->tremap = 0.0;
+    s->loop_started = false;
+    s->products_valid = false;
+    s->zmid_blocks = 0;
+    return SPK_OK;
+}
+
+int reduce_final(spk_solver s, int nparts, double scale, double *out) {
+    k_final_sum<<<1, 1024, 0, s->ctx->stream>>>(s->part, nparts, scale, s->res);
+    SPK_LAUNCH_CHECK();
+    SPK_HIP(hipMemcpyAsync(out, s->res, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SPK_OK;
+}
+
+unsigned red_grid(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b < RED_BLOCKS ? (b > 0 ? b : 1) : RED_BLOCKS);
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+
+#define SPK_CHECK_SOLVER(s) SPK_REQUIRE((s) != nullptr, SPK_ERR_ARG, "%s: null solver", __func__)
+
+extern "C" int spk_solver_create(spk_ctx ctx, const spk_params *p, int bulk, spk_solver *out) {
+    SPK_REQUIRE(ctx && p && out, SPK_ERR_ARG, "spk_solver_create: null argument");
+    SPK_HIP(hipSetDevice(ctx->device));
+    spk_solver s = new spk_solver_s();
+    memset((void *)s, 0, sizeof(*s));
+    int r = create(ctx, p, bulk, s);
+    if (r != SPK_OK) {
+        free_all(s);
+        delete s;
+        return r;
+    }
+    *out = s;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_destroy(spk_solver s) {
+    if (!s) return SPK_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_all(s);
+    delete s;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_info(spk_solver s, int *vars, size_t *ntc) {
+    SPK_CHECK_SOLVER(s);
+    if (vars) {
+        const int v[10] = {s->VX, s->VY, s->VZ, s->BX, s->BY, s->BZ, s->TH, s->VEL, s->MAG, s->nf};
+        memcpy(vars, v, sizeof(v));
+    }
+    if (ntc) *ntc = (size_t)s->g.ntc;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_buffers(spk_solver s, void **fields, void **dfields, void **realbuf) {
+    SPK_CHECK_SOLVER(s);
+    if (fields) *fields = s->U;
+    if (dfields) *dfields = s->dU;
+    if (realbuf) *realbuf = s->realbuf;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_get_time(spk_solver s, double *out8) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(out8, SPK_ERR_ARG, "spk_solver_get_time: null out");
+    TimeState h;
+    SPK_TRY(read_time(s, &h));
+    const double o[8] = {h.t, h.dt, h.tremap, h.dt_hyp, h.dt_par, h.dt_sts, h.t_lastsnap, h.step};
+    memcpy(out8, o, sizeof(o));
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_set_time(spk_solver s, double t, int step, double t_lastsnap) {
+    SPK_CHECK_SOLVER(s);
+    s->step = step;
+    k_set_time<<<1, 1, 0, s->ctx->stream>>>(s->ts, t, (double)step, t_lastsnap, s->tremap);
+    SPK_LAUNCH_CHECK();
+    s->products_valid = false;
+    return SPK_OK;
+}
+
+extern "C" int spk_clean_divergence(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    s->products_valid = false;
+    return clean_divergence(s);
+}
+
+extern "C" int spk_complex2real(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    const Grid3 &g = s->g;
+    PassArgs a{s->U, s->realbuf, s->nf, g.ntc, g.ntc, 1.0 / ((double)g.nx * g.ny * g.nz)};
+    SPK_TRY(launch_pass_x(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, a));
+    PassArgs b{s->realbuf, s->realbuf, s->nf, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, b));
+    return launch_pass_z_c2r(s->ctx, g, s->tw64, SPK_F64, SPK_F64, b);
+}
+
+extern "C" int spk_compute_dt(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    return compute_dt(s);
+}
+
+extern "C" int spk_compute_dfield(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    const double ks = s->tremap * s->p.shear;
+    SPK_TRY(rhs_front(s, ks, false));
+    s->products_valid = false;
+    return rhs_back(s, ks, 0, 0.0, 0.0);
+}
+
+extern "C" int spk_rk3(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    return rk3(s);
+}
+
+extern "C" int spk_rkl2_cycle(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    return rkl2_cycle(s);
+}
+
+extern "C" int spk_hydro_mhd_advance(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    return hydro_mhd_advance(s);
+}
+
+extern "C" int spk_main_loop(spk_solver s, int nsteps) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(nsteps >= 0, SPK_ERR_ARG, "spk_main_loop: negative step count");
+    if (!s->loop_started) {
+        // supervisor.cu:171-175
+        if (s->p.shearing) {
+            TimeState h;
+            SPK_TRY(read_time(s, &h));
+            SPK_TRY(shift_time_host(s, h.t));
+            SPK_TRY(clock(s, 0, 0, 0, 1, s->tremap));
+        }
+        s->loop_started = true;
+    }
+    for (int n = 0; n < nsteps; n++) {
+        SPK_TRY(clean_divergence(s));
+        s->products_valid = false;
+        SPK_TRY(hydro_mhd_advance(s));
+        SPK_TRY(clock(s, 0, 0, 1, 0, 0.0));
+    }
+    return SPK_OK;
+}
+
+extern "C" int spk_energy(spk_solver s, int field, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(field >= 0 && field < s->nf && out, SPK_ERR_ARG, "spk_energy: bad field %d", field);
+    const unsigned nb = red_grid(s->g.ntc);
+    k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(s->U + (long long)field * s->g.ntc, s->g.ntc, s->kg0, 0, s->part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)s->g.nx * s->g.ny * s->g.nz;
+    return reduce_final(s, (int)nb, 1.0 / (nt * nt), out);
+}
+
+extern "C" int spk_enstrophy(spk_solver s, int f0, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(f0 >= 0 && f0 + 2 < s->nf && out, SPK_ERR_ARG, "spk_enstrophy: bad field %d", f0);
+    const unsigned nb = red_grid(s->g.ntc);
+    k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(s->U + (long long)f0 * s->g.ntc, s->g.ntc,
+                                               kgeom(s, s->tremap * s->p.shear), 1, s->part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)s->g.nx * s->g.ny * s->g.nz;
+    return reduce_final(s, (int)nb, 1.0 / (nt * nt), out);
+}
+
+extern "C" int spk_correlation(spk_solver s, int fa, int fb, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out, SPK_ERR_ARG, "spk_correlation: bad fields");
+    const long long rows = (long long)s->g.nx * s->g.ny;
+    const unsigned nb = red_grid(rows * s->g.nz);
+    const long long fs = 2 * s->g.ntc;
+    k_real_dot<<<nb, 256, 0, s->ctx->stream>>>(s->realbuf + fa * fs, s->realbuf + fb * fs, rows, s->g.nz, 2 * s->g.nzh,
+                                               s->part);
+    SPK_LAUNCH_CHECK();
+    return reduce_final(s, (int)nb, 1.0 / ((double)s->g.nx * s->g.ny * s->g.nz), out);
+}
+
+extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out && cap >= 0, SPK_ERR_ARG,
+                "spk_spectrum: bad arguments");
+    const int nb = s->nbins;
+    if (nbins) *nbins = nb;
+    SPK_REQUIRE(nb <= 16384, SPK_ERR_ARG, "spk_spectrum: %d bins exceed the LDS histogram", nb);
+    k_zero_d<<<1, 256, 0, s->ctx->stream>>>(s->spec, nb + 1);
+    const unsigned g = red_grid(s->g.ntc);
+    k_spectrum<<<g, 256, sizeof(double) * (size_t)(nb + 1), s->ctx->stream>>>(
+        s->U + (long long)fa * s->g.ntc, s->U + (long long)fb * s->g.ntc, s->g.ntc, kgeom(s, s->tremap * s->p.shear),
+        (double)s->g.nx * s->g.ny * s->g.nz, s->deltak, nb, s->spec);
+    SPK_LAUNCH_CHECK();
+    std::vector<double> h((size_t)nb + 1);
+    SPK_HIP(hipMemcpyAsync(h.data(), s->spec, sizeof(double) * (size_t)(nb + 1), hipMemcpyDeviceToHost, s->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    for (int i = 0; i < nb && i < cap; i++) out[i] = h[i];
+    return SPK_OK;
+}

@@ -1,0 +1,77 @@
+"""Initial conditions in the solver's spectral layout (numpy; host side).
+
+The spectral convention is the reference's: a field's spectrum is the
+unnormalized 3D R2C transform of its real values (numpy rfftn), shape
+[NX, NY, NZ/2+1], with the real grid x_i = -L/2 + i L/N (initflow.cu:32-34).
+"""
+import numpy as np
+
+
+def grid_coords(nx, ny, nz, lx, ly, lz):
+    x = -lx / 2 + lx * np.arange(nx) / nx
+    y = -ly / 2 + ly * np.arange(ny) / ny
+    z = -lz / 2 + lz * np.arange(nz) / nz
+    return np.meshgrid(x, y, z, indexing="ij")
+
+
+def wavenumbers(nx, ny, nz, lx, ly, lz):
+    kx = 2 * np.pi / lx * np.fft.fftfreq(nx, 1.0 / nx)
+    ky = 2 * np.pi / ly * np.fft.fftfreq(ny, 1.0 / ny)
+    kz = 2 * np.pi / lz * np.arange(nz // 2 + 1)
+    return np.meshgrid(kx, ky, kz, indexing="ij")
+
+
+def dealias_mask(nx, ny, nz, lx, ly, lz):
+    """The 2/3 rule of wavevector.cu:79-93: zero where |k_i| > 2/3 * 2pi/L_i (N_i/2 - 1)."""
+    KX, KY, KZ = wavenumbers(nx, ny, nz, lx, ly, lz)
+    m = np.ones(KX.shape)
+    for K, n, L in ((KX, nx, lx), (KY, ny, ly), (KZ, nz, lz)):
+        m[np.abs(K) > 2.0 / 3.0 * (2 * np.pi / L * (n // 2 - 1))] = 0.0
+    return m
+
+
+def taylor_green(n, L=2 * np.pi, with_b=False):
+    """u = (sin x cos y cos z, -cos x sin y cos z, 0) on an n^3, L-periodic box;
+    returns the spectral state [3 or 6, n, n, n//2+1]."""
+    X, Y, Z = grid_coords(n, n, n, L, L, L)
+    k = 2 * np.pi / L
+    u = [np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z), -np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z), 0 * X]
+    fields = u + ([0 * X, 0 * X, 0 * X] if with_b else [])
+    return np.stack([np.fft.rfftn(f) for f in fields])
+
+
+def random_solenoidal(shape, L=(2 * np.pi,) * 3, seed=0, k0=4.0, rms=1.0, ncomp=3):
+    """Random-phase, solenoidal, dealiased vector field with shell spectrum
+    E(k) ~ k^4 exp(-2 (k/k0)^2), normalized to sqrt(<|u|^2>) = rms.
+    Returns the spectral components [ncomp=3, NX, NY, NZ/2+1] (complex128)."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    KX, KY, KZ = wavenumbers(nx, ny, nz, *L)
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    Kabs = np.sqrt(K2)
+    c = [np.fft.rfftn(rng.standard_normal(shape)) for _ in range(ncomp)]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.where(K2 > 0, np.sqrt(Kabs ** 4 * np.exp(-2 * (Kabs / k0) ** 2) / (4 * np.pi * np.maximum(K2, 1e-300))), 0.0)
+        ik2 = np.where(K2 > 0, 1.0 / np.maximum(K2, 1e-300), 0.0)
+    c = [a * amp for a in c]
+    # solenoidal projection
+    div = KX * c[0] + KY * c[1] + KZ * c[2]
+    c = [c[0] - KX * div * ik2, c[1] - KY * div * ik2, c[2] - KZ * div * ik2]
+    # zero the Nyquist planes and apply the 2/3 mask
+    m = dealias_mask(nx, ny, nz, *L)
+    m[nx // 2, :, :] = 0.0
+    m[:, ny // 2, :] = 0.0
+    m[:, :, nz // 2] = 0.0
+    c = [a * m for a in c]
+    # Hermitian-consistent: go through real space once
+    r = [np.fft.irfftn(a, s=shape, axes=(0, 1, 2)) for a in c]
+    cur = np.sqrt(np.mean(sum(x ** 2 for x in r)))
+    r = [x * (rms / cur) for x in r]
+    return np.stack([np.fft.rfftn(x) for x in r])
+
+
+def mhd_random(shape, L=(2 * np.pi,) * 3, seed=0, k0=4.0):
+    """u and B independent random-phase solenoidal fields, rms 1 each (state [6, ...])."""
+    u = random_solenoidal(shape, L, seed=seed, k0=k0)
+    b = random_solenoidal(shape, L, seed=seed + 1000003, k0=k0)
+    return np.concatenate([u, b])

@@ -1,0 +1,145 @@
+"""GPU parity of the fused solver (libspk, through its C ABI) against the FP64
+oracle on identical seeded inputs.
+
+Tolerances (relative L2 over all fields): FP64 bulk <= 1e-11 for one RHS,
+<= 1e-10 after several steps; FP32 bulk <= 3e-5 for one RHS, <= 2e-4 after
+several steps (FFTs and products in FP32, state in FP64)."""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from physics_cases import CASES, initial_state, relerr
+from spkmod import spk
+
+pytestmark = pytest.mark.gpu
+
+TOL_RHS = {"f64": 1e-11, "f32": 3e-5}
+TOL_STEPS = {"f64": 1e-10, "f32": 2e-4}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = spk.Ctx(0)
+    yield c
+    c.close()
+
+
+def make_pair(ctx, name, prec, seed=1):
+    p = CASES[name]["params"]
+    sim = orc.Sim(**p)
+    sol = spk.Solver(ctx, prec, **p)
+    assert sol.nf == sim.nf and sol.vars == sim.vars
+    u0 = initial_state(name, seed)
+    sim.set_fields(u0)
+    sol.set_fields(u0)
+    return sim, sol, u0
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_compute_dfield(ctx, name, prec):
+    sim, sol, _ = make_pair(ctx, name, prec)
+    sim.complex2real()
+    sim.compute_dfield(0)
+    sol.compute_dfield()
+    ref = sim.dfields.copy()
+    got = sol.get_dfields()
+    assert relerr(got, ref) < TOL_RHS[prec], (name, prec, relerr(got, ref))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_compute_dt(ctx, name):
+    sim, sol, _ = make_pair(ctx, name, "f64")
+    sim.compute_dt()
+    sol.compute_dt()
+    a, b = sim.time(), sol.time()
+    for k in ("dt", "dt_hyp", "dt_par", "dt_sts"):
+        if np.isfinite(a[k]) and a[k] != 0.0:
+            assert abs(b[k] - a[k]) <= 1e-12 * abs(a[k]), (name, k, a[k], b[k])
+        else:
+            assert a[k] == b[k] or (np.isinf(a[k]) and np.isinf(b[k])), (name, k, a[k], b[k])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_main_loop_steps(ctx, name, prec):
+    sim, sol, _ = make_pair(ctx, name, prec)
+    nsteps = 6
+    sim.main_loop(nsteps)
+    sol.main_loop(nsteps)
+    e = relerr(sol.get_fields(), sim.fields)
+    assert e < TOL_STEPS[prec], (name, prec, e)
+    ta, tb = sim.time(), sol.time()
+    assert abs(tb["t"] - ta["t"]) <= 1e-12 * max(abs(ta["t"]), 1e-300) * (1 if prec == "f64" else 1e5)
+    assert tb["step"] == ta["step"] == nsteps
+
+
+def test_shearing_remap_crossing(ctx):
+    """Start just before the remap time so the first steps cross it
+    (hydro_mhd_advance.cu:168-180, shearing_remapping.cu:15-59)."""
+    name = "shear_mhd"
+    sim, sol, _ = make_pair(ctx, name, "f64")
+    p = CASES[name]["params"]
+    t0 = p["ly"] / (2 * p["shear"] * p["lx"]) - 0.004
+    sim.set_time(t0)
+    sol.set_time(t0)
+    sim.main_loop(5)
+    sol.main_loop(5)
+    assert relerr(sol.get_fields(), sim.fields) < 1e-10
+    assert abs(sol.time()["tremap"] - sim.time()["tremap"]) < 1e-12
+    assert sim.time()["tremap"] < 0  # the remap happened
+
+
+def test_diagnostics(ctx):
+    sim, sol, _ = make_pair(ctx, "mhd", "f64")
+    v = sim.vars
+    for f in range(sim.nf):
+        assert abs(sol.energy(f) - sim.energy(f)) <= 1e-12 * abs(sim.energy(f))
+    for f0 in (v["VEL"], v["MAG"]):
+        assert abs(sol.enstrophy(f0) - sim.enstrophy(f0)) <= 1e-12 * abs(sim.enstrophy(f0))
+    sim.complex2real()
+    sol.complex2real()
+    assert relerr(sol.get_realbuf(), sim.realbuf) < 1e-13
+    for fa, fb in ((v["VX"], v["BX"]), (v["VY"], v["VY"]), (v["VZ"], v["BY"])):
+        ref = sim.correlation(fa, fb)
+        assert abs(sol.correlation(fa, fb) - ref) <= 1e-12 * max(abs(ref), 1e-3)
+    for fa, fb in ((v["VX"], v["VX"]), (v["BY"], v["BZ"])):
+        ref = sim.spectrum(fa, fb)
+        got = sol.spectrum(fa, fb)
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_rkl2_cycle(ctx):
+    name = "mhd_bouss_rkl2"
+    sim, sol, _ = make_pair(ctx, name, "f64")
+    sim.compute_dt()
+    sol.compute_dt()
+    sim.rkl2_cycle()
+    sol.rkl2_cycle()
+    assert relerr(sol.get_fields(), sim.fields) < 1e-12
+    s, _ = orc.rkl2_coeffs(sim.time()["dt"], sim.time()["dt_sts"])
+    assert s > 2  # the cycle really is multi-stage here
+
+
+def test_taylor_green_64_100_steps(ctx):
+    """Config 1: HD Taylor-Green, 64^3, 2pi box, nu=0.01, CFL 0.5, 2/3 dealiasing,
+    100 steps; kinetic energy and dissipation 2 nu <w^2> against the oracle."""
+    ini = __import__("importlib").import_module("spooky-git_amd.initial")
+    p = dict(nx=64, ny=64, nz=64, incompressible=1, lx=2 * np.pi, ly=2 * np.pi, lz=2 * np.pi, nu=0.01,
+             cfl=0.5, t_final=1e9, toutput_flow=1e9)
+    u0 = ini.taylor_green(64)
+    sim = orc.Sim(**p)
+    sol = spk.Solver(ctx, "f64", **p)
+    sim.set_fields(u0)
+    sol.set_fields(u0)
+    for chunk in range(4):
+        sim.main_loop(25)
+        sol.main_loop(25)
+        ev_ref = sum(sim.energy(f) for f in range(3))
+        ev = sum(sol.energy(f) for f in range(3))
+        eps_ref = 2 * 0.01 * sim.enstrophy(0)
+        eps = 2 * 0.01 * sol.enstrophy(0)
+        assert abs(ev - ev_ref) <= 1e-12 * ev_ref, (chunk, ev, ev_ref)
+        assert abs(eps - eps_ref) <= 1e-12 * eps_ref, (chunk, eps, eps_ref)
+    assert sol.time()["step"] == 100
