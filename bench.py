@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--precisions", default="f64,f32")
+    ap.add_argument("--mode", choices=["full", "passes"], default="full",
+                    help="passes: only the FFT pass kernels and a copy kernel (for counter runs)")
     return ap.parse_args()
 
 
@@ -94,6 +96,41 @@ def time_fft_passes(spk, torch, ctx, prec, n, reps=20):
     return out
 
 
+KERNEL_NAMES = {
+    "inv_x": "k_inv_x<double,256> (inverse x pass from the FP64 state, 6 fields)",
+    "inv_y": "k_pass_strided<double,256,+1,1> (inverse y pass, 6 fields)",
+    "zmid": "k_zmid<double,256,U|B> (z C2R of 6 fields, products, z R2C of 8)",
+    "fwd_y": "k_pass_strided<double,256,-1,1> (forward y pass, 8 fields)",
+    "fwd_x": "k_pass_strided<double,256,-1,0> (forward x pass, 8 fields)",
+    "epilogue": "k_epilogue<double,MHD,stage 2> (RHS + RK3 stage update)",
+    "clean_div": "k_clean_div (CleanFieldDivergence)",
+}
+
+
+def kernel_table(spk, ctx, state, n, prec, reps=10):
+    """Live per-kernel average launch time (HIP events, spk_kernel_timings) and
+    algorithmic HBM bytes per launch for the MHD step (6 state fields, 8 products)."""
+    sol = spk.Solver(ctx, prec, **mhd_params(n))
+    sol.set_fields(state)
+    sol.main_loop(1)
+    t = sol.kernel_timings(reps)
+    sol.destroy()
+    s64 = n * n * (n // 2 + 1) * 16
+    sb = s64 if prec == "f64" else s64 // 2
+    nin, nout, nf = 6, 8, 6
+    algo = {"inv_x": nin * (s64 + sb), "inv_y": nin * 2 * sb, "zmid": (nin + nout) * sb, "fwd_y": nout * 2 * sb,
+            "fwd_x": nout * 2 * sb, "epilogue": nout * sb + 4 * nf * s64, "clean_div": 2 * nf * s64}
+    per_step = {"inv_x": 3, "inv_y": 3, "zmid": 3, "fwd_y": 3, "fwd_x": 3, "epilogue": 3, "clean_div": 1}
+    out = {}
+    for k, b in algo.items():
+        ms = t[k]
+        out[k] = dict(ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9, per_step=per_step[k])
+    tot = sum(v["ms"] * v["per_step"] for v in out.values())
+    for v in out.values():
+        v["share"] = v["ms"] * v["per_step"] / tot
+    return out
+
+
 def copy_gbs(torch, nbytes):
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -139,6 +176,11 @@ def main():
     stream = torch.cuda.current_stream()
     ctx = spk.Ctx(0, stream.cuda_stream)
     n = a.n
+    if a.mode == "passes":
+        passes = time_fft_passes(spk, torch, ctx, "f64", n, reps=5)
+        copy = copy_gbs(torch, n * n * (n // 2 + 1) * 16 * 8)
+        print(json.dumps({"fft_pass": passes, "copy_gbs": copy, "copy_bytes": n * n * (n // 2 + 1) * 16 * 8}))
+        return
     state = ini.mhd_random((n, n, n), seed=2026)
 
     res = {}
@@ -152,18 +194,21 @@ def main():
     for k, v in passes.items():
         print("# pass %s f64: %.3f ms, %.0f GB/s" % (k, v["ms"], v["gbs"]), file=sys.stderr, flush=True)
     copy = copy_gbs(torch, n * n * (n // 2 + 1) * 16 * 8)
-    # dominant pass kernel: the forward y pass on the 8 product fields (the
-    # strided pass with the largest share of the step in the kernel trace)
-    dom = passes["y_fwd"]
+    kern = kernel_table(spk, ctx, state, n, "f64")
+    for k, v in kern.items():
+        print("# kernel %-9s f64: %8.3f ms/launch x%d per step, %6.0f GB/s, %4.1f%% of step" %
+              (k, v["ms"], v["per_step"], v["gbs"], 100 * v["share"]), file=sys.stderr, flush=True)
+    dom_name = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["per_step"])
+    dom = kern[dom_name]
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_y_fwd_f64_256.json")
+    tpath = os.path.join(REPO, "profiles", "traffic_%s_f64_256.json" % dom_name)
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roofline = dict(bound="hbm", achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="k_pass_strided<double,256,-1> (y pass, 8 fields per launch)",
-                    algorithmic_bytes_per_launch=dom["bytes"], avg_launch_ms=round(dom["ms"], 5),
+                    kernel=KERNEL_NAMES[dom_name], algorithmic_bytes_per_launch=dom["bytes"],
+                    avg_launch_ms=round(dom["ms"], 5), share_of_step=round(dom["share"], 4),
                     copy_gbs_measured=round(copy, 1))
 
     cpu = None
@@ -194,6 +239,8 @@ def main():
                        "rel_energy_diff_vs_f64": abs(res["f32"]["energy"] - f64["energy"]) / abs(f64["energy"])}
                       if "f32" in res and "f64" in res else None),
         "fft_pass_gbs_f64": {k: round(v["gbs"], 1) for k, v in passes.items()},
+        "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
+                        for k, v in kern.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -124,6 +124,16 @@ int spk_hydro_mhd_advance(spk_solver s);
  * snapshot clock of CheckSnapshot (check_output.cu:62-94) kept on the device */
 int spk_main_loop(spk_solver s, int nsteps);
 
+/* Per-kernel timing of the RK3 pipeline, each kernel class launched reps times
+ * exactly as inside a step, timed with HIP events on the context stream (the
+ * reference times every FFT with a cudaEvent pair, cufft_routines.cu:47-53).
+ * out_ms[k] = average ms per launch for k = 0 inverse x pass (from the state),
+ * 1 inverse y pass, 2 fused z kernel, 3 forward y pass, 4 forward x pass,
+ * 5 spectral epilogue (RK3 stage 2), 6 CleanFieldDivergence, 7 dt; -1 where a
+ * class does not run for this configuration.  *count = 8.  The state is left
+ * unchanged; the RK3 scratch is overwritten.  Synchronizes. */
+int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int cap, int *count);
+
 /* ---- measurement (src/io/spooky_output_functions.cu); results synchronize ---- */
 int spk_energy(spk_solver s, int field, double *out);            /* computeEnergy :23-55 */
 int spk_enstrophy(spk_solver s, int vec_first_field, double *out); /* computeEnstrophy :57-86 */

@@ -97,6 +97,7 @@ SIGNATURES = {
     "spk_rkl2_cycle": [_VP],
     "spk_hydro_mhd_advance": [_VP],
     "spk_main_loop": [_VP, _I],
+    "spk_kernel_timings": [_VP, _I, _VP, _I, _VP],
     "spk_energy": [_VP, _I, _VP],
     "spk_enstrophy": [_VP, _I, _VP],
     "spk_correlation": [_VP, _I, _I, _VP],
@@ -312,6 +313,15 @@ class Solver:
 
     def main_loop(self, nsteps):
         check(lib().spk_main_loop(self.h, nsteps))
+
+    KERNEL_CLASSES = ["inv_x", "inv_y", "zmid", "fwd_y", "fwd_x", "epilogue", "clean_div", "dt"]
+
+    def kernel_timings(self, reps=10):
+        """Average ms per launch of each pipeline kernel class (None where it does not run)."""
+        out = (ctypes.c_double * 8)()
+        n = ctypes.c_int()
+        check(lib().spk_kernel_timings(self.h, reps, out, 8, ctypes.byref(n)))
+        return {k: (v if v >= 0 else None) for k, v in zip(self.KERNEL_CLASSES, list(out))}
 
     # measurement
     def energy(self, f):

@@ -181,10 +181,25 @@ __device__ __forceinline__ cpx<T> tw_load(const cpx<T>* __restrict__ tw, int e) 
     return w;
 }
 
+// Orders LDS traffic between the threads that share a line.  WAVE: all of
+// them sit in one wavefront, whose LDS instructions execute in issue order, so
+// a wavefront-scope fence plus a code-motion barrier suffices; otherwise a
+// workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lsync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Stockham line FFT, natural layout in and out.  lds: per-line region of
-// lds_len(N) elements.  Caller guarantees every thread of the block executes
-// this (barriers inside).
-template <class T, int N, int E, int DIR>
+// lds_len(N) elements.  Caller guarantees every thread of the block (WAVE:
+// of the wavefront) executes this (barriers inside).
+template <class T, int N, int E, int DIR, bool WAVE = false>
 struct LineFFT {
     static_assert(N % E == 0, "E must divide N");
     static constexpr int TPL = N / E;                  // threads per line
@@ -206,10 +221,10 @@ struct LineFFT {
             const int idxD = (t / Ns) * Ns * E + jm;
 #pragma unroll
             for (int r = 0; r < E; r++) lds[lpad(idxD + r * Ns)] = v[r];
-            __syncthreads();
+            lsync<WAVE>();
 #pragma unroll
             for (int i = 0; i < E; i++) v[i] = lds[lpad(t + i * TPL)];
-            __syncthreads();
+            lsync<WAVE>();
         }
         if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw);
     }

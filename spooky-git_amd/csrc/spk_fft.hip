@@ -25,7 +25,8 @@ namespace spk {
 
 
 // Line FFT over columns: element (f, o, n, c) at f*fstride + o*ostride + n*lstride + c.
-template <class T, int N, int DIR>
+// AXIS (0 = x, 1 = y) only names the instantiation so profiles tell the passes apart.
+template <class T, int N, int DIR, int AXIS>
 __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_pass_strided(
     const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
     int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale) {
@@ -168,7 +169,7 @@ static const cpx<T> *tw_ptr(const Twiddles &tw, long long off) {
 
 #define SPK_POW2_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
 
-template <class T, int DIR>
+template <class T, int DIR, int AXIS>
 static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride, long long ostride,
                           int nouter, const cpx<T> *twp, const PassArgs &a) {
 #define SPK_STRIDED_CASE(NN)                                                                       \
@@ -176,7 +177,7 @@ static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride
         constexpr int TC = strided_tcols(NN);                                                      \
         const int ntiles = (int)((ncols + TC - 1) / TC);                                           \
         dim3 grid((unsigned)ntiles * nouter, a.nfields);                                          \
-        k_pass_strided<T, NN, DIR><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
+        k_pass_strided<T, NN, DIR, AXIS><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
             (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, ntiles, a.fstride_in, \
             a.fstride_out, twp, (T)a.scale);                                                       \
         break;                                                                                     \
@@ -192,28 +193,28 @@ static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride
     return SPK_OK;
 }
 
-template <int DIR>
+template <int DIR, int AXIS>
 static int pass_dispatch(spk_ctx ctx, int n, long long ncols, long long lstride, long long ostride,
                          int nouter, const Twiddles &tw, long long off, int tin, int tout,
                          const PassArgs &a) {
     SPK_REQUIRE(tin == tout && tin == tw.prec, SPK_ERR_ARG, "pass precision mismatch");
     if (tw.prec == SPK_F64)
-        return launch_strided<double, DIR>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<double>(tw, off), a);
-    return launch_strided<float, DIR>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<float>(tw, off), a);
+        return launch_strided<double, DIR, AXIS>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<double>(tw, off), a);
+    return launch_strided<float, DIR, AXIS>(ctx, n, ncols, lstride, ostride, nouter, tw_ptr<float>(tw, off), a);
 }
 
 int launch_pass_x(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
                   const PassArgs &a) {
     const long long cols = (long long)g.ny * g.nzh;
-    if (dir < 0) return pass_dispatch<-1>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
-    return pass_dispatch<+1>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
+    if (dir < 0) return pass_dispatch<-1, 0>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
+    return pass_dispatch<+1, 0>(ctx, g.nx, cols, cols, 0, 1, tw, tw.off_x, tin, tout, a);
 }
 
 int launch_pass_y(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, int tin, int tout,
                   const PassArgs &a) {
     const long long plane = (long long)g.ny * g.nzh;
-    if (dir < 0) return pass_dispatch<-1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
-    return pass_dispatch<+1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
+    if (dir < 0) return pass_dispatch<-1, 1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
+    return pass_dispatch<+1, 1>(ctx, g.ny, g.nzh, g.nzh, plane, g.nx, tw, tw.off_y, tin, tout, a);
 }
 
 #define SPK_Z_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)

@@ -20,6 +20,10 @@ template <class T, int M, int E>
 struct ZRow {
     static constexpr int TPL = M / E;
     static constexpr int LL = lds_len(M + 1);  // LDS elements per row
+    // the threads of a row share one wavefront: wavefront-level ordering only
+    static constexpr bool WAVE = TPL <= 64;
+    using FFTI = LineFFT<T, M, E, +1, WAVE>;
+    using FFTF = LineFFT<T, M, E, -1, WAVE>;
 
     // complex row src[0..M] -> z values v (natural layout).  All threads of the
     // block call this; inactive rows pass active=false.
@@ -41,7 +45,7 @@ struct ZRow {
                 L[lpad(M)] = a;
             }
         }
-        __syncthreads();
+        lsync<WAVE>();
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const int k = t + i * TPL;
@@ -51,8 +55,8 @@ struct ZRow {
             const cpx<T> wd = cmulc(d, tw_nz[k]);  // W^{-k} d
             v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
         }
-        __syncthreads();
-        LineFFT<T, M, E, +1>::run(v, L, t, tw_m);
+        lsync<WAVE>();
+        FFTI::run(v, L, t, tw_m);
     }
 
     // z values v (natural layout) -> complex row dst[0..M].  Ends with a barrier
@@ -61,10 +65,10 @@ struct ZRow {
     __device__ static __forceinline__ void r2c(cpx<T> (&v)[E], bool active, cpx<TO> *dst, cpx<T> *L, int t,
                                                const cpx<T> *__restrict__ tw_nz,
                                                const cpx<T> *__restrict__ tw_m) {
-        LineFFT<T, M, E, -1>::run(v, L, t, tw_m);
+        FFTF::run(v, L, t, tw_m);
 #pragma unroll
         for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
-        __syncthreads();
+        lsync<WAVE>();
         if (active) {
             const T h = (T)0.5;
 #pragma unroll
@@ -80,7 +84,7 @@ struct ZRow {
                 dst[M] = cconv<TO>(cpx<T>{z0.x - z0.y, (T)0});
             }
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
 };
 
