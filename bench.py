@@ -36,8 +36,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--precisions", default="f64,f32")
-    ap.add_argument("--mode", choices=["full", "passes"], default="full",
-                    help="passes: only the FFT pass kernels and a copy kernel (for counter runs)")
+    ap.add_argument("--mode", choices=["full", "passes", "kernels"], default="full",
+                    help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
+                         "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs)")
     return ap.parse_args()
 
 
@@ -50,6 +51,7 @@ def mhd_params(n):
 def time_steps(spk, torch, ctx, state, prec, n, steps, warmup):
     sol = spk.Solver(ctx, prec, **mhd_params(n))
     sol.set_fields(state)
+    fused_used = sol.pipeline()[0]
     sol.main_loop(warmup)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -62,7 +64,7 @@ def time_steps(spk, torch, ctx, state, prec, n, steps, warmup):
     ev = sum(sol.energy(f) for f in range(3))
     em = sum(sol.energy(f) for f in range(3, 6))
     sol.destroy()
-    return ms, t, ev + em
+    return ms, t, ev + em, fused_used
 
 
 def time_fft_passes(spk, torch, ctx, prec, n, reps=20):
@@ -104,12 +106,14 @@ KERNEL_NAMES = {
     "fwd_x": "k_pass_strided<double,256,-1,0> (forward x pass, 8 fields)",
     "epilogue": "k_epilogue<double,MHD,stage 2> (RHS + RK3 stage update)",
     "clean_div": "k_clean_div (CleanFieldDivergence)",
+    "fwdx_epilogue": "k_fwdx_epilogue<double,256,MHD,stage 2> (forward x pass + RHS + RK3 update + next inverse x)",
 }
 
 
-def kernel_table(spk, ctx, state, n, prec, reps=10):
+def kernel_table(spk, ctx, state, n, prec, fused, reps=10):
     """Live per-kernel average launch time (HIP events, spk_kernel_timings) and
-    algorithmic HBM bytes per launch for the MHD step (6 state fields, 8 products)."""
+    algorithmic HBM bytes per launch for the MHD step (6 state fields, 8 products);
+    fused: the step runs the fused forward-x/epilogue kernel (autotune's choice)."""
     sol = spk.Solver(ctx, prec, **mhd_params(n))
     sol.set_fields(state)
     sol.main_loop(1)
@@ -119,8 +123,14 @@ def kernel_table(spk, ctx, state, n, prec, reps=10):
     sb = s64 if prec == "f64" else s64 // 2
     nin, nout, nf = 6, 8, 6
     algo = {"inv_x": nin * (s64 + sb), "inv_y": nin * 2 * sb, "zmid": (nin + nout) * sb, "fwd_y": nout * 2 * sb,
-            "fwd_x": nout * 2 * sb, "epilogue": nout * sb + 4 * nf * s64, "clean_div": 2 * nf * s64}
-    per_step = {"inv_x": 3, "inv_y": 3, "zmid": 3, "fwd_y": 3, "fwd_x": 3, "epilogue": 3, "clean_div": 1}
+            "fwd_x": nout * 2 * sb, "epilogue": nout * sb + 4 * nf * s64, "clean_div": 2 * nf * s64,
+            "fwdx_epilogue": nout * sb + 4 * nf * s64 + nin * sb}
+    if fused:
+        # fused pipeline: forward x + epilogue + projection + next inverse x in one kernel
+        per_step = {"inv_y": 3, "zmid": 3, "fwd_y": 3, "fwdx_epilogue": 3}
+    else:
+        per_step = {"inv_x": 3, "inv_y": 3, "zmid": 3, "fwd_y": 3, "fwd_x": 3, "epilogue": 3, "clean_div": 1}
+    algo = {k: v for k, v in algo.items() if k in per_step}
     out = {}
     for k, b in algo.items():
         ms = t[k]
@@ -182,19 +192,28 @@ def main():
         print(json.dumps({"fft_pass": passes, "copy_gbs": copy, "copy_bytes": n * n * (n // 2 + 1) * 16 * 8}))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
+    if a.mode == "kernels":
+        sol = spk.Solver(ctx, "f64", **mhd_params(n))
+        sol.set_fields(state)
+        sol.main_loop(1)
+        t = sol.kernel_timings(2)
+        copy = copy_gbs(torch, n * n * (n // 2 + 1) * 16 * 8)
+        print(json.dumps({"kernel_ms": t, "pipeline": sol.pipeline(), "copy_gbs": copy,
+                          "copy_bytes": n * n * (n // 2 + 1) * 16 * 8}))
+        return
 
     res = {}
     for prec in a.precisions.split(","):
-        ms, t, e = time_steps(spk, torch, ctx, state, prec, n, a.steps, a.warmup)
-        res[prec] = dict(ms=ms, steps_per_s=1000.0 / ms, t=t["t"], dt=t["dt"], energy=e)
-        print("# %s: %.3f ms/step, %.1f steps/s, t=%.4g, E=%.6g" % (prec, ms, 1000.0 / ms, t["t"], e),
-              file=sys.stderr, flush=True)
+        ms, t, e, fz = time_steps(spk, torch, ctx, state, prec, n, a.steps, a.warmup)
+        res[prec] = dict(ms=ms, steps_per_s=1000.0 / ms, t=t["t"], dt=t["dt"], energy=e, fused=fz)
+        print("# %s: %.3f ms/step, %.1f steps/s, t=%.4g, E=%.6g, fused back end: %s" %
+              (prec, ms, 1000.0 / ms, t["t"], e, fz), file=sys.stderr, flush=True)
 
     passes = time_fft_passes(spk, torch, ctx, "f64", n)
     for k, v in passes.items():
         print("# pass %s f64: %.3f ms, %.0f GB/s" % (k, v["ms"], v["gbs"]), file=sys.stderr, flush=True)
     copy = copy_gbs(torch, n * n * (n // 2 + 1) * 16 * 8)
-    kern = kernel_table(spk, ctx, state, n, "f64")
+    kern = kernel_table(spk, ctx, state, n, "f64", res["f64"]["fused"] if "f64" in res else False)
     for k, v in kern.items():
         print("# kernel %-9s f64: %8.3f ms/launch x%d per step, %6.0f GB/s, %4.1f%% of step" %
               (k, v["ms"], v["per_step"], v["gbs"], 100 * v["share"]), file=sys.stderr, flush=True)

@@ -124,13 +124,26 @@ int spk_hydro_mhd_advance(spk_solver s);
  * snapshot clock of CheckSnapshot (check_output.cu:62-94) kept on the device */
 int spk_main_loop(spk_solver s, int nsteps);
 
+/* The fused back end of an RK3 stage runs the forward x pass, the spectral
+ * epilogue, the RK3 update, the next step's CleanFieldDivergence and the next
+ * stage's inverse x pass in one kernel; spk_main_loop can use it where the
+ * configuration allows (no stratification, shearing, sts or in-RHS
+ * conduction; 32 <= NX <= 1024).  mode 0: never; 1 (default): when the tile
+ * autotune run at spk_solver_create found it faster than the unfused kernels
+ * on this grid; 2: always where allowed.  All give the same results to rounding. */
+int spk_solver_set_fused(spk_solver s, int mode);
+/* Which back end spk_main_loop will run: *fused = 1 for the fused kernel;
+ * *points = points per thread of its x line (the autotuned tile). */
+int spk_solver_pipeline(spk_solver s, int *fused, int *points);
+
 /* Per-kernel timing of the RK3 pipeline, each kernel class launched reps times
  * exactly as inside a step, timed with HIP events on the context stream (the
  * reference times every FFT with a cudaEvent pair, cufft_routines.cu:47-53).
  * out_ms[k] = average ms per launch for k = 0 inverse x pass (from the state),
  * 1 inverse y pass, 2 fused z kernel, 3 forward y pass, 4 forward x pass,
- * 5 spectral epilogue (RK3 stage 2), 6 CleanFieldDivergence, 7 dt; -1 where a
- * class does not run for this configuration.  *count = 8.  The state is left
+ * 5 spectral epilogue (RK3 stage 2), 6 CleanFieldDivergence, 7 dt, 8 the fused
+ * forward x pass + epilogue + next inverse x pass (RK3 stage 2); -1 where a
+ * class does not run for this configuration.  *count = 9.  The state is left
  * unchanged; the RK3 scratch is overwritten.  Synchronizes. */
 int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int cap, int *count);
 

@@ -98,6 +98,8 @@ SIGNATURES = {
     "spk_hydro_mhd_advance": [_VP],
     "spk_main_loop": [_VP, _I],
     "spk_kernel_timings": [_VP, _I, _VP, _I, _VP],
+    "spk_solver_set_fused": [_VP, _I],
+    "spk_solver_pipeline": [_VP, _VP, _VP],
     "spk_energy": [_VP, _I, _VP],
     "spk_enstrophy": [_VP, _I, _VP],
     "spk_correlation": [_VP, _I, _I, _VP],
@@ -314,13 +316,23 @@ class Solver:
     def main_loop(self, nsteps):
         check(lib().spk_main_loop(self.h, nsteps))
 
-    KERNEL_CLASSES = ["inv_x", "inv_y", "zmid", "fwd_y", "fwd_x", "epilogue", "clean_div", "dt"]
+    KERNEL_CLASSES = ["inv_x", "inv_y", "zmid", "fwd_y", "fwd_x", "epilogue", "clean_div", "dt", "fwdx_epilogue"]
+
+    def set_fused(self, mode):
+        """0: unfused kernels; 1: autotuned choice (default); 2: fused kernel wherever allowed."""
+        check(lib().spk_solver_set_fused(self.h, int(mode)))
+
+    def pipeline(self):
+        """(fused back end used by main_loop, points per thread of its x line)."""
+        f, pts = ctypes.c_int(), ctypes.c_int()
+        check(lib().spk_solver_pipeline(self.h, ctypes.byref(f), ctypes.byref(pts)))
+        return bool(f.value), pts.value
 
     def kernel_timings(self, reps=10):
         """Average ms per launch of each pipeline kernel class (None where it does not run)."""
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 9)()
         n = ctypes.c_int()
-        check(lib().spk_kernel_timings(self.h, reps, out, 8, ctypes.byref(n)))
+        check(lib().spk_kernel_timings(self.h, reps, out, 9, ctypes.byref(n)))
         return {k: (v if v >= 0 else None) for k, v in zip(self.KERNEL_CLASSES, list(out))}
 
     # measurement

@@ -431,4 +431,198 @@ __global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, 
     }
 }
 
+// ------------------------------------------------------------------------
+// fused forward x pass + spectral epilogue + RK3 update (+ stage-3
+// CleanFieldDivergence + the next stage's inverse x pass).  One block row per
+// field group (blockIdx.y): velocity (products T_xx..T_yz), magnetic (emf),
+// temperature (u theta).  After each product's x line FFT its contribution to
+// k.T, k x emf or k.(u theta) is accumulated in registers, so the transformed
+// products never go back to HBM.  The groups run in different blocks and update
+// their fields in place, so no group may read another group's state: used
+// without stratification, shearing, sts or in-RHS conduction.
+// ------------------------------------------------------------------------
+// EF: points per thread of the x line (4 or 8, chosen per grid by the solver's
+// tile autotune); TPL = N/EF threads per line, TCOLS columns per 256-thread block
+__host__ __device__ constexpr int fx_e(int n, int ef) { return line_e(n, ef); }
+__host__ __device__ constexpr int fx_tpl(int n, int ef) { return n / fx_e(n, ef); }
+__host__ __device__ constexpr int fx_tcols(int n, int ef) { return fx_tpl(n, ef) >= 256 ? 1 : 256 / fx_tpl(n, ef); }
+
+struct FusedConst {
+    EpiConst e;
+    int project;    // apply CleanDivergence to the updated u and B (stage 3)
+    int inv;        // write the next stage's inverse x pass of the updated fields to W
+    double scale;   // 1/(NX NY NZ) of the next inverse pass
+};
+
+template <class T, int N, int LY, int MODE, int EF>
+__global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
+    const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S, cpx<T> *__restrict__ W,
+    FusedConst fc, const cpx<T> *__restrict__ tw, const TimeState *__restrict__ ts) {
+    using L = Layout<LY>;
+    constexpr int E = fx_e(N, EF);
+    constexpr int TPL = N / E;
+    constexpr int TC = fx_tcols(N, EF);
+    constexpr int LL = lds_len(N);
+    constexpr int VX = L::VX, BX = L::BX, TH = L::TH;
+    __shared__ cpx<T> lds[E != N ? TC * LL : 1];
+    const EpiConst &c = fc.e;
+    const long long N0 = c.ntc;
+    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    const int tid = threadIdx.x;
+    const int cl = tid % TC, t = tid / TC;
+    const long long col = (long long)xcd_remap(blockIdx.x, gridDim.x) * TC + cl;
+    const bool active = col < ncols;
+    const int j = active ? (int)(col / c.kg.nzh) : 0;
+    const int kk = active ? (int)(col - (long long)j * c.kg.nzh) : 0;
+    // group: 0 velocity, then magnetic (MHD), then temperature (boussinesq)
+    int grp = blockIdx.y;
+    if (!L::MHD && grp >= 1) grp = 2;
+    const int nprod = grp == 0 ? 5 : 3;
+    const int p0 = grp == 0 ? c.oT : (grp == 1 ? c.oE : c.oQ);
+    cpx<T> *ldsc = lds + cl * LL;
+
+    cpx<double> acc[3][E];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int i = 0; i < E; i++) acc[q][i] = cpx<double>{0.0, 0.0};
+
+    // the next product's line is loaded while the current one is transformed
+    cpx<T> vn[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) vn[i] = active ? P[(long long)p0 * N0 + col + (long long)(t + i * TPL) * ncols] : cpx<T>{0, 0};
+#pragma unroll 1
+    for (int q = 0; q < nprod; q++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = vn[i];
+        if (q + 1 < nprod) {
+            const cpx<T> *src = P + (long long)(p0 + q + 1) * N0 + col;
+#pragma unroll
+            for (int i = 0; i < E; i++) vn[i] = active ? src[(long long)(t + i * TPL) * ncols] : cpx<T>{0, 0};
+        }
+        LineFFT<T, N, E, -1>::run(v, ldsc, t, tw);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            double kx, ky, kz, m;
+            kvec_at(c.kg, t + i * TPL, j, kk, kx, ky, kz, m);
+            const cpx<double> a = cconv<double>(v[i]);
+            const cpx<double> ax = cscale(a, kx), ay = cscale(a, ky), az = cscale(a, kz);
+            if (grp == 0) {
+                // NonLinHydroAdv terms: k.T with T_zz = -T_xx - T_yy
+                switch (q) {
+                    case 0: acc[0][i] = cadd(acc[0][i], ax); acc[2][i] = cadd(acc[2][i], cscale(az, -1.0)); break;
+                    case 1: acc[0][i] = cadd(acc[0][i], ay); acc[1][i] = cadd(acc[1][i], ax); break;
+                    case 2: acc[0][i] = cadd(acc[0][i], az); acc[2][i] = cadd(acc[2][i], ax); break;
+                    case 3: acc[1][i] = cadd(acc[1][i], ay); acc[2][i] = cadd(acc[2][i], cscale(az, -1.0)); break;
+                    default: acc[1][i] = cadd(acc[1][i], az); acc[2][i] = cadd(acc[2][i], ay); break;
+                }
+            } else if (grp == 1) {
+                // MagneticShear: k x emf
+                switch (q) {
+                    case 0: acc[1][i] = cadd(acc[1][i], az); acc[2][i] = cadd(acc[2][i], cscale(ay, -1.0)); break;
+                    case 1: acc[0][i] = cadd(acc[0][i], cscale(az, -1.0)); acc[2][i] = cadd(acc[2][i], ax); break;
+                    default: acc[0][i] = cadd(acc[0][i], ay); acc[1][i] = cadd(acc[1][i], cscale(ax, -1.0)); break;
+                }
+            } else {
+                // NonLinBoussinesqAdv: k.(u theta)
+                acc[0][i] = cadd(acc[0][i], q == 0 ? ax : (q == 1 ? ay : az));
+            }
+        }
+    }
+
+    // per point: sources, diffusion, projection, RK3 update
+    const double dt = ts->dt;
+    const double ga = c.gam * dt, xb = c.xi * dt;
+    const int f0 = grp == 0 ? VX : (grp == 1 ? BX : TH);
+    const int ncomp = grp == 2 ? 1 : 3;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = t + i * TPL;
+        const long long pi = (long long)n * ncols + col;
+        double kx, ky, kz, m;
+        kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        cpx<double> d[3], u[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            u[q] = cpx<double>{0.0, 0.0};
+            if (active && q < ncomp) u[q] = U[(f0 + q) * N0 + pi];
+        }
+        if (grp == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = mul_minus_im(m, acc[q][i]);
+            const double f = -c.nu * k2;
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, VX + q, d[q], u[q], f);
+            // GradPseudoPressure
+            const cpx<double> sd3 = kdot3(kx, d[0], ky, d[1], kz, d[2]);
+            const cpx<double> div = {-sd3.y, sd3.x};
+            const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+            const double ax = kx * ik2, ay = ky * ik2, az = kz * ik2;
+            d[0] = cadd(d[0], cpx<double>{-ax * div.y, ax * div.x});
+            d[1] = cadd(d[1], cpx<double>{-ay * div.y, ay * div.x});
+            d[2] = cadd(d[2], cpx<double>{-az * div.y, az * div.x});
+        } else if (grp == 1) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = mul_plus_im(m, acc[q][i]);
+            const double f = -c.nu_m * k2;
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, BX + q, d[q], u[q], f);
+        } else {
+            d[0] = mul_minus_im(m, acc[0][i]);
+            d[0] = diffuse(c.diff_mask, TH, d[0], u[0], -c.nu_th * k2);
+            d[1] = d[2] = cpx<double>{0.0, 0.0};
+        }
+        cpx<double> un[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            un[q] = cpx<double>{0.0, 0.0};
+            if (active && q < ncomp) {
+                const long long a = (f0 + q) * N0 + pi;
+                const cpx<double> base = (MODE == 1) ? u[q] : S[a];
+                un[q] = cpx<double>{base.x + ga * d[q].x, base.y + ga * d[q].y};
+                if constexpr (MODE < 3) S[a] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
+            }
+        }
+        if (fc.project && grp < 2) {
+            // CleanDivergence of the new u / B (the next step's CleanFieldDivergence)
+            const cpx<double> sd3 = kdot3(kx, un[0], ky, un[1], kz, un[2]);
+            const cpx<double> q0 = {-sd3.y, sd3.x};
+            const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+            const double kq[3] = {kx, ky, kz};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const cpx<double> ikq = {-kq[q] * q0.y, kq[q] * q0.x};
+                un[q] = cpx<double>{un[q].x + ikq.x * ik2, un[q].y + ikq.y * ik2};
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (active && q < ncomp) U[(f0 + q) * N0 + pi] = un[q];
+            acc[q][i] = un[q];
+        }
+        // one point at a time: keeps the state loads of later points from being
+        // hoisted (they would double the live registers)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (fc.inv) {
+        // the next stage's inverse x pass of the updated fields (k_inv_x without derivatives)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (q < ncomp) {
+                cpx<T> v[E];
+#pragma unroll
+                for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
+                LineFFT<T, N, E, +1>::run(v, ldsc, t, tw);
+                if (active) {
+                    cpx<T> *dst = W + (long long)(f0 + q) * N0 + col;
+#pragma unroll
+                    for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace spk

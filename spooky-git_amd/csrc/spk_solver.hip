@@ -361,6 +361,12 @@ __global__ void k_zero_d(double *p, int n) {
 
 }  // namespace spk
 
+namespace spk {
+bool fused_supported_nx(int nx);
+int launch_fwdx_epilogue(spk_ctx ctx, int bulk, int nx, int ly, int mode, int ef, const void *P, void *U, void *S,
+                         void *W, const FusedConst &fc, const void *tw, const TimeState *ts, int ngroups);
+}  // namespace spk
+
 // ==========================================================================
 // host side
 // ==========================================================================
@@ -398,6 +404,9 @@ struct spk_solver_s {
     int step;
     double tremap;
     bool loop_started, products_valid;
+    int fused_mode;      // caller's switch for k_fwdx_epilogue: 0 never, 1 autotuned, 2 always
+    bool fused_faster;   // tile autotune: the fused back end beat the unfused one on this grid
+    int fused_ef;        // points per thread of its x line
 };
 
 namespace {
@@ -533,18 +542,11 @@ int rhs_front(spk_solver s, double kxshift, bool with_max) {
     return SPK_OK;
 }
 
-// forward transforms of the products and the spectral epilogue
-int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
-    const Grid3 &g = s->g;
-    if (s->nout > 0) {
-        PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
-        SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
-        SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
-    }
+EpiConst make_epi(spk_solver s, double kxshift, double gam, double xi) {
     EpiConst c;
     memset(&c, 0, sizeof(c));
     c.kg = kgeom(s, kxshift);
-    c.ntc = g.ntc;
+    c.ntc = s->g.ntc;
     c.stratification = s->p.stratification;
     c.sd = s->p.strat_direction;
     c.shearing = s->p.shearing;
@@ -556,8 +558,66 @@ int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
     c.oT = s->oT; c.oE = s->oE; c.oQ = s->oQ; c.oA = s->oA;
     c.gam = gam;
     c.xi = xi;
+    return c;
+}
+
+// forward transforms of the products and the spectral epilogue
+int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
+    const Grid3 &g = s->g;
+    if (s->nout > 0) {
+        PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
+        SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+        SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+    }
+    const EpiConst c = make_epi(s, kxshift, gam, xi);
     if (s->bulk == SPK_F64) return launch_epilogue<double>(s, c, mode);
     return launch_epilogue<float>(s, c, mode);
+}
+
+// ---- fused path: forward y pass, then forward x + epilogue + RK3 (+ projection
+// + next inverse x) in one kernel; the next stage starts at the inverse y pass ----
+
+// Each field group of k_fwdx_epilogue updates its own fields in place, so the
+// groups must not read each other's state: stratification (u_s <-> theta) is
+// excluded, as are shearing (sheared k between stages, remap), sts and the
+// in-RHS conduction (its grad theta slots come from k_inv_x).
+bool fused_allowed(spk_solver s) {
+    return s->nin > 0 && !s->aniso_rhs && !s->p.shearing && s->nsts == 0 &&
+           !(s->p.boussinesq && s->p.stratification) && fused_supported_nx(s->p.nx) && s->ly != LY_HEAT;
+}
+
+bool fused_ok(spk_solver s) {
+    return fused_allowed(s) && (s->fused_mode == 2 || (s->fused_mode == 1 && s->fused_faster));
+}
+
+int fused_groups(spk_solver s) { return 1 + (s->p.mhd ? 1 : 0) + (s->p.boussinesq ? 1 : 0); }
+
+int rhs_back_fused(spk_solver s, int mode, double gam, double xi, int project, int inv) {
+    const Grid3 &g = s->g;
+    PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+    FusedConst fc;
+    memset(&fc, 0, sizeof(fc));
+    fc.e = make_epi(s, 0.0, gam, xi);
+    fc.project = project;
+    fc.inv = inv;
+    fc.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+    const void *tw = (const char *)s->twb.d + (size_t)s->twb.off_x * (s->bulk == SPK_F64 ? 16 : 8);
+    return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, mode, s->fused_ef, s->P, s->U, s->S, s->W, fc, tw,
+                                s->ts, fused_groups(s));
+}
+
+// inverse y pass + fused z kernel from a W that already holds the inverse x pass
+int rhs_front_from_w(spk_solver s, bool with_max) {
+    const Grid3 &g = s->g;
+    PassArgs a{s->W, s->W, s->nin, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    double *pm = with_max ? s->pmax : nullptr;
+    unsigned nb = 0;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, s->zprog, s->W, s->P, pm, &nb));
+    else SPK_TRY(launch_zmid<float>(s, s->zprog, s->W, s->P, pm, &nb));
+    s->zmid_blocks = (int)nb;
+    return SPK_OK;
 }
 
 int dt_kernel(spk_solver s) {
@@ -753,6 +813,93 @@ int hydro_mhd_advance(spk_solver s) {
     return clock(s, 0, 1, 0, 0, 0.0);
 }
 
+// Tile autotune of the back end of one RK3 stage on this grid: the fused kernel
+// with 4 or 8 points per thread against the unfused forward x pass + epilogue +
+// next inverse x pass (+ a third of the projection pass).  Zero state, zero
+// products and a zero step keep every buffer at zero.
+int tune_fused(spk_solver s) {
+    s->fused_ef = 4;
+    s->fused_faster = false;
+    if (!fused_allowed(s)) return SPK_OK;
+    const Grid3 &g = s->g;
+    hipStream_t st = s->ctx->stream;
+    const size_t fb = (size_t)g.ntc * esize(s->bulk);
+    SPK_HIP(hipMemsetAsync(s->P, 0, fb * s->nout, st));
+    SPK_HIP(hipMemsetAsync(s->W, 0, fb * s->nin, st));
+    FusedConst fc;
+    memset(&fc, 0, sizeof(fc));
+    fc.e = make_epi(s, 0.0, 0.0, 0.0);
+    fc.inv = 1;
+    fc.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+    const void *tw = (const char *)s->twb.d + (size_t)s->twb.off_x * (s->bulk == SPK_F64 ? 16 : 8);
+    hipEvent_t e0, e1;
+    SPK_HIP(hipEventCreate(&e0));
+    SPK_HIP(hipEventCreate(&e1));
+    auto time_it = [&](auto &&launch, float *ms) -> int {
+        for (int r = 0; r < 4; r++) {
+            if (r == 1) SPK_HIP(hipEventRecord(e0, st));
+            SPK_TRY(launch());
+        }
+        SPK_HIP(hipEventRecord(e1, st));
+        SPK_HIP(hipEventSynchronize(e1));
+        SPK_HIP(hipEventElapsedTime(ms, e0, e1));
+        return SPK_OK;
+    };
+    float best = 1e30f, t_unf = 0.f, t_cl = 0.f;
+    int rc = SPK_OK;
+    for (int ef : {4, 8}) {
+        float ms = 0.f;
+        rc = time_it([&] {
+            return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, 2, ef, s->P, s->U, s->S, s->W, fc, tw, s->ts,
+                                        fused_groups(s));
+        }, &ms);
+        if (rc != SPK_OK) break;
+        if (ms < best) {
+            best = ms;
+            s->fused_ef = ef;
+        }
+    }
+    if (rc == SPK_OK) {
+        const EpiConst c = make_epi(s, 0.0, 0.0, 0.0);
+        PassArgs ap{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
+        rc = time_it([&] {
+            SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, ap));
+            SPK_TRY(s->bulk == SPK_F64 ? launch_epilogue<double>(s, c, 2) : launch_epilogue<float>(s, c, 2));
+            return s->bulk == SPK_F64 ? launch_inv_x<double>(s, s->inv, s->nin, s->W, 0.0)
+                                      : launch_inv_x<float>(s, s->inv, s->nin, s->W, 0.0);
+        }, &t_unf);
+    }
+    if (rc == SPK_OK) rc = time_it([&] { return clean_divergence(s); }, &t_cl);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SPK_TRY(rc);
+    s->fused_faster = best < t_unf + t_cl / 3.0f;
+    return SPK_OK;
+}
+
+// One main-loop iteration on the fused path.  w_valid: W holds the inverse x
+// pass of the current, already projected state (written by the previous
+// step's stage 3); otherwise CleanFieldDivergence and the inverse x pass run first.
+int step_fused(spk_solver s, bool w_valid) {
+    if (!w_valid) {
+        SPK_TRY(clean_divergence(s));
+        if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv, s->nin, s->W, 0.0));
+        else SPK_TRY(launch_inv_x<float>(s, s->inv, s->nin, s->W, 0.0));
+    }
+    // compute_dt: the stage-1 transforms carry the CFL maxima
+    SPK_TRY(rhs_front_from_w(s, true));
+    SPK_TRY(dt_kernel(s));
+    SPK_TRY(rhs_back_fused(s, 1, gammaRK[0], xiRK[0], 0, 1));
+    SPK_TRY(rhs_front_from_w(s, false));
+    SPK_TRY(rhs_back_fused(s, 2, gammaRK[1], xiRK[1], 0, 1));
+    SPK_TRY(rhs_front_from_w(s, false));
+    SPK_TRY(rhs_back_fused(s, 3, gammaRK[2], 0.0, 1, 1));
+    SPK_TRY(clock(s, 1, 0, 0, 0, 0.0));
+    s->step += 1;
+    s->products_valid = false;
+    return clock(s, 0, 1, 0, 0, 0.0);
+}
+
 void free_all(spk_solver s) {
     void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->rUc0, s->rUc1, s->rdU0,
                     s->ts, s->pmax, s->part, s->res, s->spec};
@@ -918,8 +1065,9 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     s->tremap = 0.0;
     s->loop_started = false;
     s->products_valid = false;
+    s->fused_mode = 1;
     s->zmid_blocks = 0;
-    return SPK_OK;
+    return tune_fused(s);
 }
 
 int reduce_final(spk_solver s, int nparts, double scale, double *out) {
@@ -1060,10 +1208,15 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
         }
         s->loop_started = true;
     }
+    const bool fused = fused_ok(s);
     for (int n = 0; n < nsteps; n++) {
-        SPK_TRY(clean_divergence(s));
-        s->products_valid = false;
-        SPK_TRY(hydro_mhd_advance(s));
+        if (fused) {
+            SPK_TRY(step_fused(s, n > 0));
+        } else {
+            SPK_TRY(clean_divergence(s));
+            s->products_valid = false;
+            SPK_TRY(hydro_mhd_advance(s));
+        }
         SPK_TRY(clock(s, 0, 0, 1, 0, 0.0));
     }
     return SPK_OK;
@@ -1125,7 +1278,7 @@ extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, 
 extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int cap, int *count) {
     SPK_CHECK_SOLVER(s);
     SPK_REQUIRE(reps >= 1 && out_ms && cap >= 0, SPK_ERR_ARG, "spk_kernel_timings: bad arguments");
-    constexpr int NK = 8;
+    constexpr int NK = 9;
     double ms[NK];
     for (int k = 0; k < NK; k++) ms[k] = -1.0;
     if (count) *count = NK;
@@ -1198,6 +1351,20 @@ extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int ca
             if (rc) break;
         }
         rc = timed(7, [&] { return dt_kernel(s); });
+        if (rc) break;
+        if (fused_allowed(s)) {
+            FusedConst fc;
+            memset(&fc, 0, sizeof(fc));
+            fc.e = c;
+            fc.project = 0;
+            fc.inv = 1;
+            fc.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+            const void *tw = (const char *)s->twb.d + (size_t)s->twb.off_x * (s->bulk == SPK_F64 ? 16 : 8);
+            rc = timed(8, [&] {
+                return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, 2, s->fused_ef, s->P, s->U, s->S, s->W, fc,
+                                            tw, s->ts, fused_groups(s));
+            });
+        }
     } while (0);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1206,5 +1373,19 @@ extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int ca
     SPK_HIP(hipStreamSynchronize(st));
     s->products_valid = false;
     for (int k = 0; k < NK && k < cap; k++) out_ms[k] = ms[k];
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_set_fused(spk_solver s, int mode) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(mode >= 0 && mode <= 2, SPK_ERR_ARG, "spk_solver_set_fused: mode %d", mode);
+    s->fused_mode = mode;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_pipeline(spk_solver s, int *fused, int *points) {
+    SPK_CHECK_SOLVER(s);
+    if (fused) *fused = fused_ok(s) ? 1 : 0;
+    if (points) *points = s->fused_ef;
     return SPK_OK;
 }

@@ -48,6 +48,21 @@ CASES = {
                                   lz=1.0, **BIG_T), fields="ub"),
     "quasi2d": dict(params=dict(nx=32, ny=16, nz=2, incompressible=1, mhd=1, nu=1e-3, nu_m=1e-3, cfl=0.5,
                                 lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T), fields="ub"),
+    "mhd_x32": dict(params=dict(nx=32, ny=16, nz=16, incompressible=1, mhd=1, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI,
+                                nu=0.01, nu_m=0.02, cfl=0.5, **BIG_T), fields="ub"),
+    "hd_x64": dict(params=dict(nx=64, ny=8, nz=16, incompressible=1, lx=TWO_PI, ly=1.0, lz=2.0, nu=0.005, cfl=0.5,
+                               **BIG_T), fields="u"),
+    "mhd_strat_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1, stratification=1,
+                                      strat_direction=0, N2=-0.2, nu=0.01, nu_m=0.01, nu_th=0.02, cfl=0.5,
+                                      lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T), fields="ubt"),
+    "hd_strat_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, stratification=1,
+                                     strat_direction=1, N2=0.5, nu=0.01, nu_th=0.01, cfl=0.5, lx=1.0, ly=1.0,
+                                     lz=1.0, **BIG_T), fields="ut"),
+    "hd_bouss_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, nu=0.01, nu_th=0.01,
+                                     cfl=0.5, lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
+    "mhd_bouss_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1, nu=0.01,
+                                      nu_m=0.01, nu_th=0.02, cfl=0.5, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T),
+                          fields="ubt"),
     "heat": dict(params=dict(nx=64, ny=2, nz=2, heat_equation=1, nu_th=1e-3, antialiasing=0, cfl=1.5,
                              lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="t"),
 }
@@ -75,3 +90,7 @@ def relerr(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# cases on which spk_main_loop takes the fused path (32 <= NX <= 1024, no shearing/sts/conduction)
+FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d"]

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import pyoracle as orc
-from physics_cases import CASES, initial_state, relerr
+from physics_cases import CASES, FUSED_CASES, initial_state, relerr
 from spkmod import spk
 
 pytestmark = pytest.mark.gpu
@@ -143,3 +143,33 @@ def test_taylor_green_64_100_steps(ctx):
         assert abs(ev - ev_ref) <= 1e-12 * ev_ref, (chunk, ev, ev_ref)
         assert abs(eps - eps_ref) <= 1e-12 * eps_ref, (chunk, eps, eps_ref)
     assert sol.time()["step"] == 100
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("name", FUSED_CASES)
+def test_fused_path_matches_unfused_and_oracle(ctx, name, prec):
+    """The fused forward-x + epilogue + projection + next-inverse-x kernel gives the
+    unfused pipeline's results (to rounding) and the oracle's, over several steps
+    (the W buffer handed from one step to the next is exercised from step 2 on)."""
+    sim, sol, u0 = make_pair(ctx, name, prec)
+    sol.set_fused(2)
+    ref2 = spk.Solver(ctx, prec, **CASES[name]["params"])
+    ref2.set_fused(0)
+    ref2.set_fields(u0)
+    nsteps = 5
+    sim.main_loop(nsteps)
+    sol.main_loop(nsteps)
+    ref2.main_loop(nsteps)
+    a, b = sol.get_fields(), ref2.get_fields()
+    assert relerr(a, b) < (1e-12 if prec == "f64" else 2e-5), relerr(a, b)
+    assert relerr(a, sim.fields) < TOL_STEPS[prec]
+    assert sol.time()["t"] == pytest.approx(sim.time()["t"], rel=1e-12 if prec == "f64" else 1e-6)
+    # two calls in a row (fresh W at the start of each call) equal one longer call
+    sol2 = spk.Solver(ctx, prec, **CASES[name]["params"])
+    sol2.set_fused(2)
+    sol2.set_fields(u0)
+    sol2.main_loop(2)
+    sol2.main_loop(3)
+    assert relerr(sol2.get_fields(), a) < (1e-13 if prec == "f64" else 1e-6)
+    t = sol.kernel_timings(2)
+    assert t["fwdx_epilogue"] is not None and t["fwdx_epilogue"] > 0
