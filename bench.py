@@ -220,10 +220,10 @@ def main():
     dom_name = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["per_step"])
     dom = kern[dom_name]
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_%s_f64_256.json" % dom_name)
+    tpath = os.path.join(REPO, "profiles", "traffic_f64_%d.json" % n)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f)["bytes_per_launch"].get(dom_name)
     roofline = dict(bound="hbm", achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel=KERNEL_NAMES[dom_name], algorithmic_bytes_per_launch=dom["bytes"],
