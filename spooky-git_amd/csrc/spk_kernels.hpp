@@ -198,6 +198,7 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     constexpr int RW = zmid_rw<T>(M);
     constexpr int NT = RW * Row::TPL;
     __shared__ cpx<T> lds[RW * Row::LL];
+    __shared__ cpx<T> twl[2 * M];  // tw_nz[0..M-1] | tw_m[0..M-1]
     __shared__ double smax[6][(NT + 63) / 64];
 
     const int tid = threadIdx.x;
@@ -206,9 +207,24 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     const bool active = row < nrows;
     cpx<T> *L = lds + rl * Row::LL;
 
+    // every input row is loaded before the first transform: one memory latency
+    // per block instead of one per field (the LDS fences inside a transform keep
+    // later loads from being issued early)
     cpx<T> in[Z::NIN][E];
+    cpx<T> nyq[Z::NIN];
 #pragma unroll
-    for (int f = 0; f < Z::NIN; f++) Row::c2r(W + f * fs + row * NZH, active, L, t, tw_nz, tw_m, (T)1, in[f]);
+    for (int f = 0; f < Z::NIN; f++) Row::load(W + f * fs + row * NZH, active, t, in[f], nyq[f]);
+    // Twiddles are read from LDS: a global twiddle load issued after the stores
+    // of a previous row transform would wait for those stores (vector memory
+    // operations retire in issue order).
+    for (int e = tid; e < M; e += NT) {
+        twl[e] = tw_nz[e];
+        twl[M + e] = tw_m[e];
+    }
+    __syncthreads();
+    const cpx<T> *tz = twl, *tm = twl + M;
+#pragma unroll
+    for (int f = 0; f < Z::NIN; f++) Row::c2r_regs(in[f], nyq[f], active, L, t, tz, tm, (T)1);
 
     if constexpr (Z::U) {
         if (pmax) {
@@ -256,7 +272,7 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
             }
             v[i] = cpx<T>{zproduct<PROG>(q, ae, zc), zproduct<PROG>(q, ao, zc)};
         }
-        Row::r2c(v, active, P + q * fs + row * NZH, L, t, tw_nz, tw_m);
+        Row::r2c(v, active, P + q * fs + row * NZH, L, t, tz, tm);
     }
 }
 
@@ -454,6 +470,191 @@ struct FusedConst {
     double scale;   // 1/(NX NY NZ) of the next inverse pass
 };
 
+// One field group of k_fwdx_epilogue: GRP 0 velocity (5 products T_ij), 1
+// magnetic (3 products emf), 2 temperature (3 products u theta).  Every lane
+// runs the same straight-line code: a lane past the last column works on the
+// last column and stores the same values as that column's own lane, so no
+// memory operation sits behind a branch.
+template <class T, int N, int LY, int MODE, int EF, int GRP>
+__device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S,
+                                         cpx<T> *__restrict__ W, const FusedConst &fc,
+                                         const cpx<T> *__restrict__ tw, cpx<T> *twl, cpx<T> *ldsc,
+                                         const TimeState *__restrict__ ts, int tid, int nt, int t, long long col) {
+    using L = Layout<LY>;
+    constexpr int E = fx_e(N, EF);
+    constexpr int TPL = N / E;
+    constexpr int NPROD = GRP == 0 ? 5 : 3;
+    constexpr int NCOMP = GRP == 2 ? 1 : 3;
+    constexpr int F0 = GRP == 0 ? L::VX : (GRP == 1 ? L::BX : L::TH);
+    const EpiConst &c = fc.e;
+    const long long N0 = c.ntc;
+    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    const int j = (int)(col / c.kg.nzh);
+    const int kk = (int)(col - (long long)j * c.kg.nzh);
+    const int p0 = GRP == 0 ? c.oT : (GRP == 1 ? c.oE : c.oQ);
+    // ky, kz are constant along the x line
+    double ky, kz;
+    {
+        double kx0, m0;
+        kvec_at(c.kg, 0, j, kk, kx0, ky, kz, m0);
+    }
+
+    cpx<double> acc[NCOMP][E];
+#pragma unroll
+    for (int q = 0; q < NCOMP; q++)
+#pragma unroll
+        for (int i = 0; i < E; i++) acc[q][i] = cpx<double>{0.0, 0.0};
+
+    // the next product's line is loaded while the current one is transformed
+    cpx<T> vn[E];
+    {
+        const cpx<T> *src = P + (long long)p0 * N0 + col;
+#pragma unroll
+        for (int i = 0; i < E; i++) vn[i] = src[(long long)(t + i * TPL) * ncols];
+    }
+    // Twiddles are read from LDS: a global twiddle load would wait for every
+    // earlier vector memory operation (they retire in issue order), i.e. for the
+    // next product's prefetch and for the state stores.
+    for (int e = tid; e < N; e += nt) twl[e] = tw[e];
+    __syncthreads();
+
+#pragma unroll 1
+    for (int q = 0; q < NPROD; q++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = vn[i];
+        if (q + 1 < NPROD) {
+            const cpx<T> *src = P + (long long)(p0 + q + 1) * N0 + col;
+#pragma unroll
+            for (int i = 0; i < E; i++) vn[i] = src[(long long)(t + i * TPL) * ncols];
+        }
+        LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
+        // coefficient of product q in each accumulated component, as a select on
+        // kx (per point) and ky, kz (per line)
+        const bool q0 = q == 0, q1 = q == 1, q2 = q == 2, q3 = q == 3, q4 = q == 4;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            double kx, kyi, kzi, m;
+            kvec_at(c.kg, t + i * TPL, j, kk, kx, kyi, kzi, m);
+            const cpx<double> a = cconv<double>(v[i]);
+            if constexpr (GRP == 0) {
+                // NonLinHydroAdv: k_j T_ij with T_zz = -T_xx - T_yy
+                const double c0 = q0 ? kx : (q1 ? ky : (q2 ? kz : 0.0));
+                const double c1 = q1 ? kx : (q3 ? ky : (q4 ? kz : 0.0));
+                const double c2 = (q0 || q3) ? -kz : (q2 ? kx : (q4 ? ky : 0.0));
+                acc[0][i] = cpx<double>{acc[0][i].x + c0 * a.x, acc[0][i].y + c0 * a.y};
+                acc[1][i] = cpx<double>{acc[1][i].x + c1 * a.x, acc[1][i].y + c1 * a.y};
+                acc[2][i] = cpx<double>{acc[2][i].x + c2 * a.x, acc[2][i].y + c2 * a.y};
+            } else if constexpr (GRP == 1) {
+                // MagneticShear: k x emf
+                const double c0 = q1 ? -kz : (q2 ? ky : 0.0);
+                const double c1 = q0 ? kz : (q2 ? -kx : 0.0);
+                const double c2 = q0 ? -ky : (q1 ? kx : 0.0);
+                acc[0][i] = cpx<double>{acc[0][i].x + c0 * a.x, acc[0][i].y + c0 * a.y};
+                acc[1][i] = cpx<double>{acc[1][i].x + c1 * a.x, acc[1][i].y + c1 * a.y};
+                acc[2][i] = cpx<double>{acc[2][i].x + c2 * a.x, acc[2][i].y + c2 * a.y};
+            } else {
+                // NonLinBoussinesqAdv: k.(u theta)
+                const double c0 = q0 ? kx : (q1 ? ky : kz);
+                acc[0][i] = cpx<double>{acc[0][i].x + c0 * a.x, acc[0][i].y + c0 * a.y};
+            }
+        }
+    }
+
+    // per point: sources, diffusion, projection, RK3 update.  The state of point
+    // i + PD is loaded while point i is computed; the schedule barrier at the end
+    // of each point keeps the compiler from hoisting all E points' loads at once.
+    const double dt = ts->dt;
+    const double ga = c.gam * dt, xb = c.xi * dt;
+    constexpr int PD = 2;
+    cpx<double> su[E][NCOMP], ss[E][NCOMP];
+    auto load_state = [&](int i) {
+        const long long pi = (long long)(t + i * TPL) * ncols + col;
+#pragma unroll
+        for (int q = 0; q < NCOMP; q++) {
+            su[i][q] = U[(F0 + q) * N0 + pi];
+            if constexpr (MODE != 1) ss[i][q] = S[(F0 + q) * N0 + pi];
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < PD && i < E; i++) load_state(i);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        if (i + PD < E) load_state(i + PD);
+        const int n = t + i * TPL;
+        const long long pi = (long long)n * ncols + col;
+        double kx, kyi, kzi, m;
+        kvec_at(c.kg, n, j, kk, kx, kyi, kzi, m);
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        cpx<double> d[NCOMP];
+        if constexpr (GRP == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = mul_minus_im(m, acc[q][i]);
+            const double f = -c.nu * k2;
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::VX + q, d[q], su[i][q], f);
+            // GradPseudoPressure
+            const cpx<double> sd3 = kdot3(kx, d[0], ky, d[1], kz, d[2]);
+            const cpx<double> div = {-sd3.y, sd3.x};
+            const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+            const double ax = kx * ik2, ay = ky * ik2, az = kz * ik2;
+            d[0] = cadd(d[0], cpx<double>{-ax * div.y, ax * div.x});
+            d[1] = cadd(d[1], cpx<double>{-ay * div.y, ay * div.x});
+            d[2] = cadd(d[2], cpx<double>{-az * div.y, az * div.x});
+        } else if constexpr (GRP == 1) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = mul_plus_im(m, acc[q][i]);
+            const double f = -c.nu_m * k2;
+#pragma unroll
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::BX + q, d[q], su[i][q], f);
+        } else {
+            d[0] = mul_minus_im(m, acc[0][i]);
+            d[0] = diffuse(c.diff_mask, L::TH, d[0], su[i][0], -c.nu_th * k2);
+        }
+        cpx<double> un[NCOMP];
+#pragma unroll
+        for (int q = 0; q < NCOMP; q++) {
+            const cpx<double> base = (MODE == 1) ? su[i][q] : ss[i][q];
+            un[q] = cpx<double>{base.x + ga * d[q].x, base.y + ga * d[q].y};
+            if constexpr (MODE < 3)
+                S[(F0 + q) * N0 + pi] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
+        }
+        if constexpr (GRP < 2) {
+            if (fc.project) {
+                // CleanDivergence of the new u / B (the next step's CleanFieldDivergence)
+                const cpx<double> sd3 = kdot3(kx, un[0], ky, un[1], kz, un[2]);
+                const cpx<double> q0 = {-sd3.y, sd3.x};
+                const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
+                const double kq[3] = {kx, ky, kz};
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const cpx<double> ikq = {-kq[q] * q0.y, kq[q] * q0.x};
+                    un[q] = cpx<double>{un[q].x + ikq.x * ik2, un[q].y + ikq.y * ik2};
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NCOMP; q++) {
+            U[(F0 + q) * N0 + pi] = un[q];
+            acc[q][i] = un[q];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (fc.inv) {
+        // the next stage's inverse x pass of the updated fields (k_inv_x without derivatives)
+#pragma unroll
+        for (int q = 0; q < NCOMP; q++) {
+            cpx<T> v[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
+            LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
+            cpx<T> *dst = W + (long long)(F0 + q) * N0 + col;
+#pragma unroll
+            for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
+        }
+    }
+}
+
 template <class T, int N, int LY, int MODE, int EF>
 __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
     const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S, cpx<T> *__restrict__ W,
@@ -463,165 +664,22 @@ __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilog
     constexpr int TPL = N / E;
     constexpr int TC = fx_tcols(N, EF);
     constexpr int LL = lds_len(N);
-    constexpr int VX = L::VX, BX = L::BX, TH = L::TH;
     __shared__ cpx<T> lds[E != N ? TC * LL : 1];
-    const EpiConst &c = fc.e;
-    const long long N0 = c.ntc;
-    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    __shared__ cpx<T> twl[N];
+    const long long ncols = (long long)fc.e.kg.ny * fc.e.kg.nzh;
     const int tid = threadIdx.x;
     const int cl = tid % TC, t = tid / TC;
-    const long long col = (long long)xcd_remap(blockIdx.x, gridDim.x) * TC + cl;
-    const bool active = col < ncols;
-    const int j = active ? (int)(col / c.kg.nzh) : 0;
-    const int kk = active ? (int)(col - (long long)j * c.kg.nzh) : 0;
-    // group: 0 velocity, then magnetic (MHD), then temperature (boussinesq)
-    int grp = blockIdx.y;
-    if (!L::MHD && grp >= 1) grp = 2;
-    const int nprod = grp == 0 ? 5 : 3;
-    const int p0 = grp == 0 ? c.oT : (grp == 1 ? c.oE : c.oQ);
+    long long col = (long long)xcd_remap(blockIdx.x, gridDim.x) * TC + cl;
+    if (col >= ncols) col = ncols - 1;
     cpx<T> *ldsc = lds + cl * LL;
-
-    cpx<double> acc[3][E];
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-#pragma unroll
-        for (int i = 0; i < E; i++) acc[q][i] = cpx<double>{0.0, 0.0};
-
-    // the next product's line is loaded while the current one is transformed
-    cpx<T> vn[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) vn[i] = active ? P[(long long)p0 * N0 + col + (long long)(t + i * TPL) * ncols] : cpx<T>{0, 0};
-#pragma unroll 1
-    for (int q = 0; q < nprod; q++) {
-        cpx<T> v[E];
-#pragma unroll
-        for (int i = 0; i < E; i++) v[i] = vn[i];
-        if (q + 1 < nprod) {
-            const cpx<T> *src = P + (long long)(p0 + q + 1) * N0 + col;
-#pragma unroll
-            for (int i = 0; i < E; i++) vn[i] = active ? src[(long long)(t + i * TPL) * ncols] : cpx<T>{0, 0};
-        }
-        LineFFT<T, N, E, -1>::run(v, ldsc, t, tw);
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            double kx, ky, kz, m;
-            kvec_at(c.kg, t + i * TPL, j, kk, kx, ky, kz, m);
-            const cpx<double> a = cconv<double>(v[i]);
-            const cpx<double> ax = cscale(a, kx), ay = cscale(a, ky), az = cscale(a, kz);
-            if (grp == 0) {
-                // NonLinHydroAdv terms: k.T with T_zz = -T_xx - T_yy
-                switch (q) {
-                    case 0: acc[0][i] = cadd(acc[0][i], ax); acc[2][i] = cadd(acc[2][i], cscale(az, -1.0)); break;
-                    case 1: acc[0][i] = cadd(acc[0][i], ay); acc[1][i] = cadd(acc[1][i], ax); break;
-                    case 2: acc[0][i] = cadd(acc[0][i], az); acc[2][i] = cadd(acc[2][i], ax); break;
-                    case 3: acc[1][i] = cadd(acc[1][i], ay); acc[2][i] = cadd(acc[2][i], cscale(az, -1.0)); break;
-                    default: acc[1][i] = cadd(acc[1][i], az); acc[2][i] = cadd(acc[2][i], ay); break;
-                }
-            } else if (grp == 1) {
-                // MagneticShear: k x emf
-                switch (q) {
-                    case 0: acc[1][i] = cadd(acc[1][i], az); acc[2][i] = cadd(acc[2][i], cscale(ay, -1.0)); break;
-                    case 1: acc[0][i] = cadd(acc[0][i], cscale(az, -1.0)); acc[2][i] = cadd(acc[2][i], ax); break;
-                    default: acc[0][i] = cadd(acc[0][i], ay); acc[1][i] = cadd(acc[1][i], cscale(ax, -1.0)); break;
-                }
-            } else {
-                // NonLinBoussinesqAdv: k.(u theta)
-                acc[0][i] = cadd(acc[0][i], q == 0 ? ax : (q == 1 ? ay : az));
-            }
-        }
-    }
-
-    // per point: sources, diffusion, projection, RK3 update
-    const double dt = ts->dt;
-    const double ga = c.gam * dt, xb = c.xi * dt;
-    const int f0 = grp == 0 ? VX : (grp == 1 ? BX : TH);
-    const int ncomp = grp == 2 ? 1 : 3;
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int n = t + i * TPL;
-        const long long pi = (long long)n * ncols + col;
-        double kx, ky, kz, m;
-        kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        cpx<double> d[3], u[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            u[q] = cpx<double>{0.0, 0.0};
-            if (active && q < ncomp) u[q] = U[(f0 + q) * N0 + pi];
-        }
-        if (grp == 0) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = mul_minus_im(m, acc[q][i]);
-            const double f = -c.nu * k2;
-#pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, VX + q, d[q], u[q], f);
-            // GradPseudoPressure
-            const cpx<double> sd3 = kdot3(kx, d[0], ky, d[1], kz, d[2]);
-            const cpx<double> div = {-sd3.y, sd3.x};
-            const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
-            const double ax = kx * ik2, ay = ky * ik2, az = kz * ik2;
-            d[0] = cadd(d[0], cpx<double>{-ax * div.y, ax * div.x});
-            d[1] = cadd(d[1], cpx<double>{-ay * div.y, ay * div.x});
-            d[2] = cadd(d[2], cpx<double>{-az * div.y, az * div.x});
-        } else if (grp == 1) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = mul_plus_im(m, acc[q][i]);
-            const double f = -c.nu_m * k2;
-#pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, BX + q, d[q], u[q], f);
-        } else {
-            d[0] = mul_minus_im(m, acc[0][i]);
-            d[0] = diffuse(c.diff_mask, TH, d[0], u[0], -c.nu_th * k2);
-            d[1] = d[2] = cpx<double>{0.0, 0.0};
-        }
-        cpx<double> un[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            un[q] = cpx<double>{0.0, 0.0};
-            if (active && q < ncomp) {
-                const long long a = (f0 + q) * N0 + pi;
-                const cpx<double> base = (MODE == 1) ? u[q] : S[a];
-                un[q] = cpx<double>{base.x + ga * d[q].x, base.y + ga * d[q].y};
-                if constexpr (MODE < 3) S[a] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
-            }
-        }
-        if (fc.project && grp < 2) {
-            // CleanDivergence of the new u / B (the next step's CleanFieldDivergence)
-            const cpx<double> sd3 = kdot3(kx, un[0], ky, un[1], kz, un[2]);
-            const cpx<double> q0 = {-sd3.y, sd3.x};
-            const double ik2 = k2 > 0.0 ? 1.0 / k2 : 1.0;
-            const double kq[3] = {kx, ky, kz};
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const cpx<double> ikq = {-kq[q] * q0.y, kq[q] * q0.x};
-                un[q] = cpx<double>{un[q].x + ikq.x * ik2, un[q].y + ikq.y * ik2};
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (active && q < ncomp) U[(f0 + q) * N0 + pi] = un[q];
-            acc[q][i] = un[q];
-        }
-        // one point at a time: keeps the state loads of later points from being
-        // hoisted (they would double the live registers)
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if (fc.inv) {
-        // the next stage's inverse x pass of the updated fields (k_inv_x without derivatives)
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (q < ncomp) {
-                cpx<T> v[E];
-#pragma unroll
-                for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
-                LineFFT<T, N, E, +1>::run(v, ldsc, t, tw);
-                if (active) {
-                    cpx<T> *dst = W + (long long)(f0 + q) * N0 + col;
-#pragma unroll
-                    for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
-                }
-            }
-        }
+    // group: 0 velocity, then magnetic (MHD), then temperature (boussinesq)
+    const int g = blockIdx.y;
+    if (g == 0) {
+        fx_group<T, N, LY, MODE, EF, 0>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
+    } else if (L::MHD && g == 1) {
+        fx_group<T, N, LY, MODE, EF, 1>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
+    } else if constexpr (L::BOUSS) {
+        fx_group<T, N, LY, MODE, EF, 2>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
     }
 }
 

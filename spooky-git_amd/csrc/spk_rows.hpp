@@ -25,22 +25,41 @@ struct ZRow {
     using FFTI = LineFFT<T, M, E, +1, WAVE>;
     using FFTF = LineFFT<T, M, E, -1, WAVE>;
 
+    // Global loads of a complex row src[0..M] into registers: v[i] = src[t + i*TPL],
+    // nyq = src[M] (thread 0 only).  Kept apart from the transform so a kernel can
+    // issue the loads of several rows before the first LDS fence.
+    template <class TI>
+    __device__ static __forceinline__ void load(const cpx<TI> *src, bool active, int t, cpx<T> (&v)[E], cpx<T> &nyq) {
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = active ? cconv<T>(src[t + i * TPL]) : cpx<T>{(T)0, (T)0};
+        nyq = (active && t == 0) ? cconv<T>(src[M]) : cpx<T>{(T)0, (T)0};
+    }
+
     // complex row src[0..M] -> z values v (natural layout).  All threads of the
     // block call this; inactive rows pass active=false.
     template <class TI>
     __device__ static __forceinline__ void c2r(const cpx<TI> *src, bool active, cpx<T> *L, int t,
                                                const cpx<T> *__restrict__ tw_nz,
                                                const cpx<T> *__restrict__ tw_m, T scale, cpx<T> (&v)[E]) {
+        cpx<T> nyq;
+        load(src, active, t, v, nyq);
+        c2r_regs(v, nyq, active, L, t, tw_nz, tw_m, scale);
+    }
+
+    // as c2r, from a row already loaded by load(): v in = row values, out = z values
+    __device__ static __forceinline__ void c2r_regs(cpx<T> (&v)[E], cpx<T> nyq, bool active, cpx<T> *L, int t,
+                                                    const cpx<T> *__restrict__ tw_nz,
+                                                    const cpx<T> *__restrict__ tw_m, T scale) {
         if (active) {
 #pragma unroll
             for (int i = 0; i < E; i++) {
                 const int k = t + i * TPL;
-                cpx<T> a = cscale(cconv<T>(src[k]), scale);
+                cpx<T> a = cscale(v[i], scale);
                 if (k == 0) a.y = (T)0;
                 L[lpad(k)] = a;
             }
             if (t == 0) {
-                cpx<T> a = cscale(cconv<T>(src[M]), scale);
+                cpx<T> a = cscale(nyq, scale);
                 a.y = (T)0;
                 L[lpad(M)] = a;
             }
