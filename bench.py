@@ -10,7 +10,7 @@ loop: CleanFieldDivergence + HydroMHDAdvance (compute_dt + 3 RK3 stages =
 
 Prints ONE JSON line.  value = FP64 steps/s; the FP32-bulk mode (FFTs and
 products in FP32, state and RK3 in FP64) is reported beside it.  roofline is
-measured live for the dominant kernel class, the FFT pass kernels, with HIP
+measured live for the kernel with the largest share of the FP64 step, with HIP
 events on the library's stream; cpu_baseline times the FP64 oracle on the host
 cores for a bounded sample of the same workload.
 """
@@ -51,7 +51,7 @@ def mhd_params(n):
 def time_steps(spk, torch, ctx, state, prec, n, steps, warmup):
     sol = spk.Solver(ctx, prec, **mhd_params(n))
     sol.set_fields(state)
-    fused_used = sol.pipeline()[0]
+    fused_used = sol.pipeline()
     sol.main_loop(warmup)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -205,9 +205,10 @@ def main():
     res = {}
     for prec in a.precisions.split(","):
         ms, t, e, fz = time_steps(spk, torch, ctx, state, prec, n, a.steps, a.warmup)
-        res[prec] = dict(ms=ms, steps_per_s=1000.0 / ms, t=t["t"], dt=t["dt"], energy=e, fused=fz)
-        print("# %s: %.3f ms/step, %.1f steps/s, t=%.4g, E=%.6g, fused back end: %s" %
-              (prec, ms, 1000.0 / ms, t["t"], e, fz), file=sys.stderr, flush=True)
+        res[prec] = dict(ms=ms, steps_per_s=1000.0 / ms, t=t["t"], dt=t["dt"], energy=e, fused=fz[0],
+                         fused_points=fz[1])
+        print("# %s: %.3f ms/step, %.1f steps/s, t=%.4g, E=%.6g, fused back end: %s (%d points/thread)" %
+              (prec, ms, 1000.0 / ms, t["t"], e, fz[0], fz[1]), file=sys.stderr, flush=True)
 
     passes = time_fft_passes(spk, torch, ctx, "f64", n)
     for k, v in passes.items():
@@ -253,7 +254,9 @@ def main():
                    "cfl": 0.5, "box": "2pi periodic",
                    "ic": "random-phase solenoidal u,B, E(k)~k^4 exp(-2(k/4)^2), rms 1 each, seed 2026",
                    "parallelism": "single GPU"},
+        "pipeline_f64": {"fused_back_end": f64["fused"], "points_per_thread": f64["fused_points"]},
         "fp32_bulk": ({"steps_per_s": round(res["f32"]["steps_per_s"], 3), "ms_per_step": round(res["f32"]["ms"], 4),
+                       "fused_back_end": res["f32"]["fused"], "points_per_thread": res["f32"]["fused_points"],
                        "ratio_vs_f64": round(res["f32"]["steps_per_s"] / f64["steps_per_s"], 3),
                        "rel_energy_diff_vs_f64": abs(res["f32"]["energy"] - f64["energy"]) / abs(f64["energy"])}
                       if "f32" in res and "f64" in res else None),

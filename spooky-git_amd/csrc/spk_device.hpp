@@ -199,25 +199,53 @@ __device__ __forceinline__ void lsync() {
 // Stockham line FFT, natural layout in and out.  lds: per-line region of
 // lds_len(N) elements.  Caller guarantees every thread of the block (WAVE:
 // of the wavefront) executes this (barriers inside).
-template <class T, int N, int E, int DIR, bool WAVE = false>
+// TWPF: each pass loads the next pass's twiddles before its LDS exchange, so
+// their latency (LDS or cache) hides behind the exchange; costs E-1 complex
+// registers.  Without it each twiddle is loaded where it is used.
+template <class T, int N, int E, int DIR, bool WAVE = false, bool TWPF = false>
 struct LineFFT {
     static_assert(N % E == 0, "E must divide N");
     static constexpr int TPL = N / E;                  // threads per line
     static constexpr int NPASS = (E > 1) ? ilog2c(N) / ilog2c(E) : 0;  // full radix-E passes
     static constexpr int B = (E > 1) ? N / ipow(E, NPASS) : 1;  // final radix (< E)
+    static constexpr int G = (B > 1) ? E / B : 1;       // radix-B groups per thread in the tail
+    static constexpr int NW = E > 1 ? E - 1 : 1;        // twiddles per thread per pass
+
+    // twiddle r (1 <= r < E) of full pass P (P >= 1)
+    template <int P>
+    __device__ static __forceinline__ cpx<T> tw_full(int t, int r, const cpx<T>* __restrict__ tw) {
+        constexpr int Ns = ipow(E, P);
+        return tw_load<DIR>(tw, (t % Ns) * r * (N / (Ns * E)));
+    }
+    // twiddle r (1 <= r < B) of tail group M
+    __device__ static __forceinline__ cpx<T> tw_tail(int t, int M, int r, const cpx<T>* __restrict__ tw) {
+        return tw_load<DIR>(tw, (t + M * TPL) * r);
+    }
 
     template <int P>
     __device__ static __forceinline__ void pass_full(cpx<T> (&v)[E], cpx<T>* lds, int t,
-                                                     const cpx<T>* __restrict__ tw) {
+                                                     const cpx<T>* __restrict__ tw, const cpx<T> (&w)[NW]) {
         constexpr int Ns = ipow(E, P);
-        const int jm = t % Ns;
         if constexpr (P > 0) {
 #pragma unroll
-            for (int r = 1; r < E; r++) v[r] = cmul(v[r], tw_load<DIR>(tw, jm * r * (N / (Ns * E))));
+            for (int r = 1; r < E; r++) v[r] = cmul(v[r], TWPF ? w[r - 1] : tw_full<P>(t, r, tw));
         }
         dft_group<E, DIR, 0, 1>(v);
+        cpx<T> wn[NW];
+        if constexpr (TWPF) {
+            if constexpr (P + 1 < NPASS) {
+#pragma unroll
+                for (int r = 1; r < E; r++) wn[r - 1] = tw_full<P + 1>(t, r, tw);
+            } else if constexpr (B > 1) {
+#pragma unroll
+                for (int m = 0; m < G; m++)
+#pragma unroll
+                    for (int r = 1; r < B; r++) wn[m * (B - 1) + r - 1] = tw_tail(t, m, r, tw);
+            }
+        }
         constexpr bool last = (P == NPASS - 1) && (B == 1);
         if constexpr (!last) {
+            const int jm = t % Ns;
             const int idxD = (t / Ns) * Ns * E + jm;
 #pragma unroll
             for (int r = 0; r < E; r++) lds[lpad(idxD + r * Ns)] = v[r];
@@ -226,27 +254,26 @@ struct LineFFT {
             for (int i = 0; i < E; i++) v[i] = lds[lpad(t + i * TPL)];
             lsync<WAVE>();
         }
-        if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw);
+        if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw, wn);
+        else if constexpr (B > 1) pass_tail<0>(v, t, tw, wn);
     }
 
     template <int M>
-    __device__ static __forceinline__ void pass_tail(cpx<T> (&v)[E], int t, const cpx<T>* __restrict__ tw) {
-        // radix-B butterflies j = t + M*TPL on registers v[M + r*(E/B)]
-        constexpr int G = E / B;
-        const int j = t + M * TPL;
+    __device__ static __forceinline__ void pass_tail(cpx<T> (&v)[E], int t, const cpx<T>* __restrict__ tw,
+                                                     const cpx<T> (&w)[NW]) {
+        // radix-B butterflies j = t + M*TPL on registers v[M + r*G]
 #pragma unroll
-        for (int r = 1; r < B; r++) v[M + r * G] = cmul(v[M + r * G], tw_load<DIR>(tw, j * r));
+        for (int r = 1; r < B; r++)
+            v[M + r * G] = cmul(v[M + r * G], TWPF ? w[M * (B - 1) + r - 1] : tw_tail(t, M, r, tw));
         dft_group<B, DIR, M, G>(v);
-        if constexpr (M + 1 < G) pass_tail<M + 1>(v, t, tw);
+        if constexpr (M + 1 < G) pass_tail<M + 1>(v, t, tw, w);
     }
 
     __device__ static __forceinline__ void run(cpx<T> (&v)[E], cpx<T>* lds, int t,
                                                const cpx<T>* __restrict__ tw) {
-        if constexpr (E == 1) {
-            return;
-        } else {
-            if constexpr (NPASS > 0) pass_full<0>(v, lds, t, tw);
-            if constexpr (B > 1) pass_tail<0>(v, t, tw);
+        if constexpr (E > 1) {
+            cpx<T> w0[NW];
+            pass_full<0>(v, lds, t, tw, w0);
         }
     }
 };

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;
     constexpr int E = zmid_e<T>(M);
-    using Row = ZRow<T, M, E>;
+    using Row = ZRow<T, M, E, true>;
     constexpr int RW = zmid_rw<T>(M);
     constexpr int NT = RW * Row::TPL;
     __shared__ cpx<T> lds[RW * Row::LL];
@@ -528,7 +528,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 #pragma unroll
             for (int i = 0; i < E; i++) vn[i] = src[(long long)(t + i * TPL) * ncols];
         }
-        LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
+        LineFFT<T, N, E, -1, false, true>::run(v, ldsc, t, twl);
         // coefficient of product q in each accumulated component, as a select on
         // kx (per point) and ky, kz (per line)
         const bool q0 = q == 0, q1 = q == 1, q2 = q == 2, q3 = q == 3, q4 = q == 4;
@@ -647,7 +647,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             cpx<T> v[E];
 #pragma unroll
             for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
-            LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
+            LineFFT<T, N, E, +1, false, true>::run(v, ldsc, t, twl);
             cpx<T> *dst = W + (long long)(F0 + q) * N0 + col;
 #pragma unroll
             for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];

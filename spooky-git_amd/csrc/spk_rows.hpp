@@ -16,14 +16,16 @@
 
 namespace spk {
 
-template <class T, int M, int E>
+// TWPF: twiddles are loaded ahead of the LDS exchange that precedes their use
+// (see LineFFT); worth it when they come from LDS or cache with long latency.
+template <class T, int M, int E, bool TWPF = false>
 struct ZRow {
     static constexpr int TPL = M / E;
     static constexpr int LL = lds_len(M + 1);  // LDS elements per row
     // the threads of a row share one wavefront: wavefront-level ordering only
     static constexpr bool WAVE = TPL <= 64;
-    using FFTI = LineFFT<T, M, E, +1, WAVE>;
-    using FFTF = LineFFT<T, M, E, -1, WAVE>;
+    using FFTI = LineFFT<T, M, E, +1, WAVE, TWPF>;
+    using FFTF = LineFFT<T, M, E, -1, WAVE, TWPF>;
 
     // Global loads of a complex row src[0..M] into registers: v[i] = src[t + i*TPL],
     // nyq = src[M] (thread 0 only).  Kept apart from the transform so a kernel can
@@ -64,6 +66,11 @@ struct ZRow {
                 L[lpad(M)] = a;
             }
         }
+        cpx<T> wz[E];
+        if constexpr (TWPF) {
+#pragma unroll
+            for (int i = 0; i < E; i++) wz[i] = tw_nz[t + i * TPL];
+        }
         lsync<WAVE>();
 #pragma unroll
         for (int i = 0; i < E; i++) {
@@ -71,7 +78,7 @@ struct ZRow {
             const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
             const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
             const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
-            const cpx<T> wd = cmulc(d, tw_nz[k]);  // W^{-k} d
+            const cpx<T> wd = cmulc(d, TWPF ? wz[i] : tw_nz[k]);  // W^{-k} d
             v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
         }
         lsync<WAVE>();
@@ -85,6 +92,11 @@ struct ZRow {
                                                const cpx<T> *__restrict__ tw_nz,
                                                const cpx<T> *__restrict__ tw_m) {
         FFTF::run(v, L, t, tw_m);
+        cpx<T> wz[E];
+        if constexpr (TWPF) {
+#pragma unroll
+            for (int i = 0; i < E; i++) wz[i] = tw_nz[t + i * TPL];
+        }
 #pragma unroll
         for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
         lsync<WAVE>();
@@ -96,7 +108,7 @@ struct ZRow {
                 const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
                 const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
                 const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
-                dst[k] = cconv<TO>(ev + cmul(od, tw_nz[k]));
+                dst[k] = cconv<TO>(ev + cmul(od, TWPF ? wz[i] : tw_nz[k]));
             }
             if (t == 0) {
                 const cpx<T> z0 = L[0];
