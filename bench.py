@@ -13,6 +13,13 @@ products in FP32, state and RK3 in FP64) is reported beside it.  roofline is
 measured live for the kernel with the largest share of the FP64 step, with HIP
 events on the library's stream; cpu_baseline times the FP64 oracle on the host
 cores for a bounded sample of the same workload.
+
+Beside the timed line (not part of `value`):
+  fp32_bulk.accuracy_vs_f64  FP32-bulk against FP64 after --accuracy-steps
+                             steps (energy, cross helicity, spectra);
+  taylor_green_64            64^3 HD Taylor-Green, 100 steps, GPU and oracle;
+  mti_rkl2                   256^3 Boussinesq MTI with anisotropic conduction,
+                             FP64 RKL2 on theta: steps/s and stages per cycle.
 """
 import argparse
 import importlib
@@ -36,6 +43,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--precisions", default="f64,f32")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP32-bulk vs FP64 accuracy run (0: skip)")
+    ap.add_argument("--mti-steps", type=int, default=10)
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the Taylor-Green 64^3 and MTI legs")
     ap.add_argument("--mode", choices=["full", "passes", "kernels"], default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
                          "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs)")
@@ -172,6 +184,111 @@ def cpu_baseline(n, state, steps):
                 seconds=dt)
 
 
+def rkl2_stages(dt, dt_sts):
+    """RKL2 stage count of one cycle for dt / dt_sts (rkl.cu:466-470)."""
+    import math
+    if not (dt_sts > 0 and math.isfinite(dt_sts)):
+        return 0
+    r = dt / dt_sts
+    return 1 + int(4.0 * (1.0 + 2.0 * r) / (1.0 + math.sqrt(9.0 + 16.0 * r)))
+
+
+def fp32_accuracy(spk, ctx, state, n, steps):
+    """FP32-bulk against FP64 after `steps` main-loop steps from the same state:
+    relative errors of total energy ev+em, cross helicity <u.B> and the six
+    shell spectra Kx..Mz (largest relative error over bins holding >= 1e-6
+    of the spectrum's peak)."""
+    out = {}
+    for prec in ("f64", "f32"):
+        sol = spk.Solver(ctx, prec, **mhd_params(n))
+        sol.set_fields(state)
+        sol.main_loop(steps)
+        e = sum(sol.energy(f) for f in range(6))
+        sol.complex2real()
+        hc = sum(sol.correlation(i, 3 + i) for i in range(3))
+        spec = [sol.spectrum(f, f) for f in range(6)]
+        out[prec] = dict(e=e, hc=hc, spec=spec, t=sol.time()["t"])
+        sol.destroy()
+    a, b = out["f64"], out["f32"]
+    serr = 0.0
+    for sa, sb in zip(a["spec"], b["spec"]):
+        keep = sa > 1e-6 * sa.max()
+        serr = max(serr, float((abs(sb[keep] - sa[keep]) / sa[keep]).max()))
+    return dict(steps=steps, rel_energy_err=abs(b["e"] - a["e"]) / abs(a["e"]),
+                rel_cross_helicity_err=abs(b["hc"] - a["hc"]) / max(abs(a["hc"]), 1e-300),
+                spectra_max_rel_err=serr, t_f64=a["t"], t_f32=b["t"])
+
+
+def taylor_green_leg(spk, ini, torch, ctx, cpu):
+    """64^3 HD Taylor-Green, nu = 0.01, CFL 0.5, 100 steps on the GPU (timed)
+    and on the FP64 oracle; kinetic energy and dissipation 2 nu <w^2> compared."""
+    import math
+    n, steps = 64, 100
+    p = dict(nx=n, ny=n, nz=n, incompressible=1, lx=2 * math.pi, ly=2 * math.pi, lz=2 * math.pi, nu=0.01,
+             cfl=0.5, t_final=1e9, toutput_flow=1e9)
+    u0 = ini.taylor_green(n)
+    sol = spk.Solver(ctx, "f64", **p)
+    sol.set_fields(u0)
+    sol.main_loop(2)
+    sol.set_fields(u0)
+    sol.set_time(0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol.main_loop(steps)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    ev = sum(sol.energy(f) for f in range(3))
+    eps = 2 * 0.01 * sol.enstrophy(0)
+    sol.destroy()
+    res = dict(steps=steps, gpu_steps_per_s=steps / gpu_s, ev=ev, eps=eps)
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle as orc
+        sim = orc.Sim(**p)
+        sim.set_fields(u0)
+        t0 = time.perf_counter()
+        sim.main_loop(steps)
+        cs = time.perf_counter() - t0
+        ev_r = sum(sim.energy(f) for f in range(3))
+        eps_r = 2 * 0.01 * sim.enstrophy(0)
+        res.update(cpu_steps_per_s=steps / cs, cpu_cores=orc.lib().orc_num_threads(),
+                   rel_err_ev=abs(ev - ev_r) / ev_r, rel_err_eps=abs(eps - eps_r) / eps_r)
+    return res
+
+
+def mti_leg(spk, ini, torch, ctx, n, steps, warmup, cpu):
+    """256^3 Boussinesq MTI with anisotropic conduction, FP64, RKL2 on theta
+    (see spooky-git_amd.initial.mti_params): steps/s with HIP events, and the
+    RKL2 stage count of the run's dt / dt_sts; the FP64 oracle times one step."""
+    p = ini.mti_params(n)
+    state = ini.mti_white_noise((n, n, n), seed=7)
+    sol = spk.Solver(ctx, "f64", **p)
+    sol.set_fields(state)
+    sol.main_loop(warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    sol.main_loop(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    tm = sol.time()
+    sol.destroy()
+    res = dict(grid=[n, n, n], steps=steps, ms_per_step=ms, steps_per_s=1000.0 / ms, dt=tm["dt"],
+               dt_sts=tm["dt_sts"], rkl2_stages=rkl2_stages(tm["dt"], tm["dt_sts"]),
+               params={k: p[k] for k in ("nu", "nu_m", "nu_th", "reynolds_ani", "N2", "OmegaT2", "lx")},
+               ic="u=0, B=(1e-4,0,0), theta white noise uniform in [-1e-4,1e-4], seed 7")
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle as orc
+        sim = orc.Sim(**p)
+        sim.set_fields(state)
+        t0 = time.perf_counter()
+        sim.main_loop(1)
+        res.update(cpu_s_per_step=time.perf_counter() - t0, cpu_cores=orc.lib().orc_num_threads())
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -231,6 +348,18 @@ def main():
                     avg_launch_ms=round(dom["ms"], 5), share_of_step=round(dom["share"], 4),
                     copy_gbs_measured=round(copy, 1))
 
+    acc = fp32_accuracy(spk, ctx, state, n, a.accuracy_steps) if a.accuracy_steps > 0 else None
+    if acc:
+        print("# fp32 vs fp64 after %d steps: energy %.2e, cross helicity %.2e, spectra %.2e" %
+              (acc["steps"], acc["rel_energy_err"], acc["rel_cross_helicity_err"], acc["spectra_max_rel_err"]),
+              file=sys.stderr, flush=True)
+    tg = taylor_green_leg(spk, ini, torch, ctx, not a.no_cpu) if a.extra else None
+    if tg:
+        print("# taylor-green 64^3: %s" % json.dumps(tg), file=sys.stderr, flush=True)
+    mti = mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu) if a.extra else None
+    if mti:
+        print("# mti %d^3: %s" % (n, json.dumps(mti)), file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, state, a.cpu_steps)
@@ -258,8 +387,12 @@ def main():
         "fp32_bulk": ({"steps_per_s": round(res["f32"]["steps_per_s"], 3), "ms_per_step": round(res["f32"]["ms"], 4),
                        "fused_back_end": res["f32"]["fused"], "points_per_thread": res["f32"]["fused_points"],
                        "ratio_vs_f64": round(res["f32"]["steps_per_s"] / f64["steps_per_s"], 3),
-                       "rel_energy_diff_vs_f64": abs(res["f32"]["energy"] - f64["energy"]) / abs(f64["energy"])}
+                       "rel_energy_diff_vs_f64_after_timed_run": abs(res["f32"]["energy"] - f64["energy"]) /
+                       abs(f64["energy"]),
+                       "accuracy_vs_f64": acc}
                       if "f32" in res and "f64" in res else None),
+        "taylor_green_64": tg,
+        "mti_rkl2": mti,
         "fft_pass_gbs_f64": {k: round(v["gbs"], 1) for k, v in passes.items()},
         "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
                         for k, v in kern.items()},

@@ -274,6 +274,13 @@ class Solver:
     def get_fields(self):
         return download_ptr(self.ctx, self.d_fields, np.complex128, (self.nf,) + self.shape_c)
 
+    def get_mode(self, field, i, j, k):
+        """Complex value of spectral mode (i, j, k) of one state field (synchronizes)."""
+        nx, ny, nzh = self.shape_c
+        off = ((field * nx + i) * ny + j) * nzh + k
+        a = download_ptr(self.ctx, ctypes.c_void_p(self.d_fields.value + 16 * off), np.complex128, (1,))
+        return complex(a[0])
+
     def get_dfields(self):
         return download_ptr(self.ctx, self.d_dfields, np.complex128, (self.nf,) + self.shape_c)
 

@@ -75,3 +75,47 @@ def mhd_random(shape, L=(2 * np.pi,) * 3, seed=0, k0=4.0):
     u = random_solenoidal(shape, L, seed=seed, k0=k0)
     b = random_solenoidal(shape, L, seed=seed + 1000003, k0=k0)
     return np.concatenate([u, b])
+
+
+# 3D Boussinesq MTI with anisotropic (Braginskii) conduction: gravity along -z,
+# background dT/dz = -1 (N2, OmegaT2 as in the reference's mti-eigenmode test,
+# tests/mti-eigenmode/spooky.cfg), weak uniform B along x.
+def mti_params(n=256, chi=1e-2, nu=1e-4, eta=1e-4, nu_th=1e-6, N2=0.1, OmegaT2=1.0, L=1.0, cfl=1.5,
+               safety_sts=100):
+    """Solver parameters of the MTI run: theta (field 6) is the sts variable,
+    advanced by RKL2; everything else by RK3."""
+    return dict(nx=n, ny=n, nz=n, incompressible=1, boussinesq=1, stratification=1, strat_direction=2, mhd=1,
+                anisotropic_diffusion=1, supertimestepping=1, sts_mask=1 << 6, safety_sts=safety_sts,
+                nu=nu, nu_m=eta, nu_th=nu_th, reynolds_ani=1.0 / chi, N2=N2, OmegaT2=OmegaT2,
+                cfl=cfl, cfl_par=1.0, cfl_sts=1.0, safety_source=0.2, lx=L, ly=L, lz=L,
+                t_final=1e9, toutput_flow=1e9)
+
+
+def mti_white_noise(shape, L=(1.0, 1.0, 1.0), seed=0, amp=1e-4, b0=1e-4):
+    """State [7, ...]: u = 0, B = (b0, 0, 0), theta = seeded real-space white
+    noise uniform in [-amp, amp], Nyquist planes zeroed and 2/3 mask applied."""
+    rng = np.random.default_rng(seed)
+    th = np.fft.rfftn(rng.uniform(-amp, amp, size=shape))
+    m = dealias_mask(*shape, *L)
+    m[shape[0] // 2, :, :] = 0.0
+    m[:, shape[1] // 2, :] = 0.0
+    m[:, :, shape[2] // 2] = 0.0
+    th = np.fft.rfftn(np.fft.irfftn(th * m, s=shape, axes=(0, 1, 2)))
+    out = np.zeros((7,) + th.shape, dtype=np.complex128)
+    out[3, 0, 0, 0] = b0 * np.prod(shape)
+    out[6] = th
+    return out
+
+
+def mti_growth_rate(kx, chi, nu, eta, b0, N2):
+    """Growth rate of the MTI eigenmode with wavevector along B (x), OmegaT2 = 1:
+    largest real root of the cubic of tests/mti-eigenmode/mti_disp_rel.py:44-70,
+    written in physical units (k2 = chi kx^2, q = eta/chi, Pr = nu/chi,
+    Lambda = b0^2/chi)."""
+    k2 = chi * kx * kx
+    q, pr, lam = eta / chi, nu / chi, b0 * b0 / chi
+    coeff = [1.0, k2 * (1 + q + pr), N2 + k2 * lam + (q + pr) * k2 ** 2 + q * pr * k2 ** 2,
+             q * pr * k2 ** 3 + N2 * q * k2 + lam * k2 ** 2 - k2]
+    r = np.roots(coeff)
+    real = np.real(r[np.abs(np.imag(r)) < 1e-9 * np.maximum(1.0, np.abs(r))])
+    return float(np.max(real)) if real.size else float("nan")
