@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, 
 // tile autotune); TPL = N/EF threads per line, TCOLS columns per 256-thread block
 __host__ __device__ constexpr int fx_e(int n, int ef) { return line_e(n, ef); }
 __host__ __device__ constexpr int fx_tpl(int n, int ef) { return n / fx_e(n, ef); }
-__host__ __device__ constexpr int fx_tcols(int n, int ef) { return fx_tpl(n, ef) >= 256 ? 1 : 256 / fx_tpl(n, ef); }
+__host__ __device__ constexpr int fx_tcols(int n, int ef) { return fx_tpl(n, ef) >= 512 ? 1 : 512 / fx_tpl(n, ef); }
 
 struct FusedConst {
     EpiConst e;
@@ -475,10 +475,21 @@ struct FusedConst {
 // runs the same straight-line code: a lane past the last column works on the
 // last column and stores the same values as that column's own lane, so no
 // memory operation sits behind a branch.
+// vmcnt(0) and lgkmcnt(0) waits (gfx9 s_waitcnt encodings; other counters at
+// their maxima)
+#define SPK_WAIT_VMCNT0() __builtin_amdgcn_s_waitcnt(0x0F70)
+#define SPK_WAIT_LGKMCNT0() __builtin_amdgcn_s_waitcnt(0xC07F)
+
+// One field group of k_fwdx_epilogue: GRP 0 velocity (5 products T_ij), 1
+// magnetic (3 products emf), 2 temperature (3 products u theta).  Every lane
+// runs the same straight-line code: a lane past the last column works on the
+// last column and stores the same values as that column's own lane, so no
+// memory operation sits behind a branch.  Element offsets inside a field fit
+// 32 bits and are added to wave-uniform field bases.
 template <class T, int N, int LY, int MODE, int EF, int GRP>
 __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S,
                                          cpx<T> *__restrict__ W, const FusedConst &fc,
-                                         const cpx<T> *__restrict__ tw, cpx<T> *twl, cpx<T> *ldsc,
+                                         const cpx<T> *__restrict__ tw, cpx<T> *twl, cpx<T> *ldsc, cpx<T> *pbuf,
                                          const TimeState *__restrict__ ts, int tid, int nt, int t, long long col) {
     using L = Layout<LY>;
     constexpr int E = fx_e(N, EF);
@@ -486,12 +497,21 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     constexpr int NPROD = GRP == 0 ? 5 : 3;
     constexpr int NCOMP = GRP == 2 ? 1 : 3;
     constexpr int F0 = GRP == 0 ? L::VX : (GRP == 1 ? L::BX : L::TH);
+    // FP64 bulk: product lines reach LDS by direct global->LDS loads
+    // (global_load_lds_dwordx4), which take no registers.  Lane l of wave w puts
+    // slot i at pbuf[(w E + i) 64 + l] and reads back only its own slots, so only
+    // the wave's own waits order them.  FP32 bulk (8-byte elements) prefetches
+    // into registers.
+    constexpr bool DMA = sizeof(cpx<T>) == 16;
     const EpiConst &c = fc.e;
     const long long N0 = c.ntc;
-    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    const unsigned ncols = (unsigned)((long long)c.kg.ny * c.kg.nzh);
+    const unsigned ucol = (unsigned)col;
     const int j = (int)(col / c.kg.nzh);
     const int kk = (int)(col - (long long)j * c.kg.nzh);
     const int p0 = GRP == 0 ? c.oT : (GRP == 1 ? c.oE : c.oQ);
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     // ky, kz are constant along the x line
     double ky, kz;
     {
@@ -505,13 +525,21 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 #pragma unroll
         for (int i = 0; i < E; i++) acc[q][i] = cpx<double>{0.0, 0.0};
 
-    // the next product's line is loaded while the current one is transformed
-    cpx<T> vn[E];
-    {
-        const cpx<T> *src = P + (long long)p0 * N0 + col;
+    cpx<T> vn[DMA ? 1 : E];
+    auto fetch = [&](int q) {
+        const cpx<T> *src = P + (long long)(p0 + q) * N0;
 #pragma unroll
-        for (int i = 0; i < E; i++) vn[i] = src[(long long)(t + i * TPL) * ncols];
-    }
+        for (int i = 0; i < E; i++) {
+            const cpx<T> *g = src + (ucol + (unsigned)(t + i * TPL) * ncols);
+            if constexpr (DMA)
+                __builtin_amdgcn_global_load_lds((const void *)g,
+                                                 (__attribute__((address_space(3))) void *)(pbuf + (wv * E + i) * 64),
+                                                 16, 0, 0);
+            else
+                vn[i] = *g;
+        }
+    };
+    fetch(0);
     // Twiddles are read from LDS: a global twiddle load would wait for every
     // earlier vector memory operation (they retire in issue order), i.e. for the
     // next product's prefetch and for the state stores.
@@ -521,21 +549,25 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 #pragma unroll 1
     for (int q = 0; q < NPROD; q++) {
         cpx<T> v[E];
+        if constexpr (DMA) {
+            SPK_WAIT_VMCNT0();
 #pragma unroll
-        for (int i = 0; i < E; i++) v[i] = vn[i];
-        if (q + 1 < NPROD) {
-            const cpx<T> *src = P + (long long)(p0 + q + 1) * N0 + col;
+            for (int i = 0; i < E; i++) v[i] = pbuf[(wv * E + i) * 64 + lane];
+            SPK_WAIT_LGKMCNT0();  // the slots are read before they are refilled
+        } else {
 #pragma unroll
-            for (int i = 0; i < E; i++) vn[i] = src[(long long)(t + i * TPL) * ncols];
+            for (int i = 0; i < E; i++) v[i] = vn[DMA ? 0 : i];
         }
-        LineFFT<T, N, E, -1, false, true>::run(v, ldsc, t, twl);
+        // the next product's line is loaded while this one is transformed
+        if (q + 1 < NPROD) fetch(q + 1);
+        LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
         // coefficient of product q in each accumulated component, as a select on
         // kx (per point) and ky, kz (per line)
         const bool q0 = q == 0, q1 = q == 1, q2 = q == 2, q3 = q == 3, q4 = q == 4;
 #pragma unroll
         for (int i = 0; i < E; i++) {
-            double kx, kyi, kzi, m;
-            kvec_at(c.kg, t + i * TPL, j, kk, kx, kyi, kzi, m);
+            const int n = t + i * TPL;
+            const double kx = c.kg.kfx * (double)(n < c.kg.nx / 2 ? n : n - c.kg.nx);  // no shear here
             const cpx<double> a = cconv<double>(v[i]);
             if constexpr (GRP == 0) {
                 // NonLinHydroAdv: k_j T_ij with T_zz = -T_xx - T_yy
@@ -566,14 +598,14 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     // of each point keeps the compiler from hoisting all E points' loads at once.
     const double dt = ts->dt;
     const double ga = c.gam * dt, xb = c.xi * dt;
-    constexpr int PD = 2;
+    constexpr int PD = 1;
     cpx<double> su[E][NCOMP], ss[E][NCOMP];
     auto load_state = [&](int i) {
-        const long long pi = (long long)(t + i * TPL) * ncols + col;
+        const unsigned pi = ucol + (unsigned)(t + i * TPL) * ncols;
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
-            su[i][q] = U[(F0 + q) * N0 + pi];
-            if constexpr (MODE != 1) ss[i][q] = S[(F0 + q) * N0 + pi];
+            su[i][q] = (U + (F0 + q) * N0)[pi];
+            if constexpr (MODE != 1) ss[i][q] = (S + (F0 + q) * N0)[pi];
         }
     };
 #pragma unroll
@@ -582,7 +614,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     for (int i = 0; i < E; i++) {
         if (i + PD < E) load_state(i + PD);
         const int n = t + i * TPL;
-        const long long pi = (long long)n * ncols + col;
+        const unsigned pi = ucol + (unsigned)n * ncols;
         double kx, kyi, kzi, m;
         kvec_at(c.kg, n, j, kk, kx, kyi, kzi, m);
         const double k2 = kx * kx + ky * ky + kz * kz;
@@ -617,7 +649,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             const cpx<double> base = (MODE == 1) ? su[i][q] : ss[i][q];
             un[q] = cpx<double>{base.x + ga * d[q].x, base.y + ga * d[q].y};
             if constexpr (MODE < 3)
-                S[(F0 + q) * N0 + pi] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
+                (S + (F0 + q) * N0)[pi] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
         }
         if constexpr (GRP < 2) {
             if (fc.project) {
@@ -635,7 +667,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         }
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
-            U[(F0 + q) * N0 + pi] = un[q];
+            (U + (F0 + q) * N0)[pi] = un[q];
             acc[q][i] = un[q];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -647,10 +679,10 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             cpx<T> v[E];
 #pragma unroll
             for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
-            LineFFT<T, N, E, +1, false, true>::run(v, ldsc, t, twl);
-            cpx<T> *dst = W + (long long)(F0 + q) * N0 + col;
+            LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
+            cpx<T> *dst = W + (long long)(F0 + q) * N0;
 #pragma unroll
-            for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
+            for (int i = 0; i < E; i++) dst[ucol + (unsigned)(t + i * TPL) * ncols] = v[i];
         }
     }
 }
@@ -664,8 +696,10 @@ __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilog
     constexpr int TPL = N / E;
     constexpr int TC = fx_tcols(N, EF);
     constexpr int LL = lds_len(N);
+    constexpr int NT = TC * TPL;
     __shared__ cpx<T> lds[E != N ? TC * LL : 1];
     __shared__ cpx<T> twl[N];
+    __shared__ cpx<T> pbuf[sizeof(cpx<T>) == 16 ? NT * E : 1];
     const long long ncols = (long long)fc.e.kg.ny * fc.e.kg.nzh;
     const int tid = threadIdx.x;
     const int cl = tid % TC, t = tid / TC;
@@ -675,11 +709,11 @@ __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilog
     // group: 0 velocity, then magnetic (MHD), then temperature (boussinesq)
     const int g = blockIdx.y;
     if (g == 0) {
-        fx_group<T, N, LY, MODE, EF, 0>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
+        fx_group<T, N, LY, MODE, EF, 0>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
     } else if (L::MHD && g == 1) {
-        fx_group<T, N, LY, MODE, EF, 1>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
+        fx_group<T, N, LY, MODE, EF, 1>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
     } else if constexpr (L::BOUSS) {
-        fx_group<T, N, LY, MODE, EF, 2>(P, U, S, W, fc, tw, twl, ldsc, ts, tid, TC * TPL, t, col);
+        fx_group<T, N, LY, MODE, EF, 2>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
     }
 }
 
