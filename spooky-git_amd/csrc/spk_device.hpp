@@ -199,10 +199,24 @@ __device__ __forceinline__ void lsync() {
 // Stockham line FFT, natural layout in and out.  lds: per-line region of
 // lds_len(N) elements.  Caller guarantees every thread of the block (WAVE:
 // of the wavefront) executes this (barriers inside).
-// TWPF: each pass loads the next pass's twiddles before its LDS exchange, so
-// their latency (LDS or cache) hides behind the exchange; costs E-1 complex
-// registers.  Without it each twiddle is loaded where it is used.
-template <class T, int N, int E, int DIR, bool WAVE = false, bool TWPF = false>
+// Twiddle modes (TWM):
+//   0  each twiddle is loaded from the table where it is used;
+//   1  each pass loads the next pass's twiddles before its LDS exchange, so
+//      their latency hides behind the exchange (E-1 extra complex registers);
+//   2  the caller loads all of a thread's twiddles once (load_tw) and passes
+//      them to run_tw: a kernel that runs many transforms of one length on
+//      the same thread layout reads the table once.
+// All twiddles one thread of a line FFT of length N (E points per thread)
+// uses, forward convention: w[P-1][r-1] for full pass P >= 1, wt[] for the tail.
+template <class T, int N, int E>
+struct LineTw {
+    static constexpr int NPASS = (E > 1) ? ilog2c(N) / ilog2c(E) : 0;
+    static constexpr int NW = E > 1 ? E - 1 : 1;
+    cpx<T> w[NPASS > 1 ? NPASS - 1 : 1][NW];
+    cpx<T> wt[NW];
+};
+
+template <class T, int N, int E, int DIR, bool WAVE = false, int TWM = 0>
 struct LineFFT {
     static_assert(N % E == 0, "E must divide N");
     static constexpr int TPL = N / E;                  // threads per line
@@ -211,6 +225,12 @@ struct LineFFT {
     static constexpr int G = (B > 1) ? E / B : 1;       // radix-B groups per thread in the tail
     static constexpr int NW = E > 1 ? E - 1 : 1;        // twiddles per thread per pass
 
+    using Tw = LineTw<T, N, E>;  // conjugated at use for DIR > 0
+
+    __device__ static __forceinline__ cpx<T> dirc(cpx<T> w) {
+        if constexpr (DIR > 0) w.y = -w.y;
+        return w;
+    }
     // twiddle r (1 <= r < E) of full pass P (P >= 1)
     template <int P>
     __device__ static __forceinline__ cpx<T> tw_full(int t, int r, const cpx<T>* __restrict__ tw) {
@@ -223,16 +243,42 @@ struct LineFFT {
     }
 
     template <int P>
+    __device__ static __forceinline__ void load_tw_pass(Tw& tr, int t, const cpx<T>* __restrict__ tw) {
+        if constexpr (P < NPASS) {
+            constexpr int Ns = ipow(E, P);
+#pragma unroll
+            for (int r = 1; r < E; r++) tr.w[P - 1][r - 1] = tw[(t % Ns) * r * (N / (Ns * E))];
+            load_tw_pass<P + 1>(tr, t, tw);
+        }
+    }
+    __device__ static __forceinline__ void load_tw(Tw& tr, int t, const cpx<T>* __restrict__ tw) {
+        if constexpr (NPASS > 1) load_tw_pass<1>(tr, t, tw);
+        if constexpr (B > 1) {
+#pragma unroll
+            for (int m = 0; m < G; m++)
+#pragma unroll
+                for (int r = 1; r < B; r++) tr.wt[m * (B - 1) + r - 1] = tw[(t + m * TPL) * r];
+        }
+    }
+
+    template <int P>
     __device__ static __forceinline__ void pass_full(cpx<T> (&v)[E], cpx<T>* lds, int t,
-                                                     const cpx<T>* __restrict__ tw, const cpx<T> (&w)[NW]) {
+                                                     const cpx<T>* __restrict__ tw, const cpx<T> (&w)[NW],
+                                                     const Tw* tr) {
         constexpr int Ns = ipow(E, P);
         if constexpr (P > 0) {
 #pragma unroll
-            for (int r = 1; r < E; r++) v[r] = cmul(v[r], TWPF ? w[r - 1] : tw_full<P>(t, r, tw));
+            for (int r = 1; r < E; r++) {
+                cpx<T> wr;
+                if constexpr (TWM == 2) wr = dirc(tr->w[P - 1][r - 1]);
+                else if constexpr (TWM == 1) wr = w[r - 1];
+                else wr = tw_full<P>(t, r, tw);
+                v[r] = cmul(v[r], wr);
+            }
         }
         dft_group<E, DIR, 0, 1>(v);
         cpx<T> wn[NW];
-        if constexpr (TWPF) {
+        if constexpr (TWM == 1) {
             if constexpr (P + 1 < NPASS) {
 #pragma unroll
                 for (int r = 1; r < E; r++) wn[r - 1] = tw_full<P + 1>(t, r, tw);
@@ -254,26 +300,39 @@ struct LineFFT {
             for (int i = 0; i < E; i++) v[i] = lds[lpad(t + i * TPL)];
             lsync<WAVE>();
         }
-        if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw, wn);
-        else if constexpr (B > 1) pass_tail<0>(v, t, tw, wn);
+        if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw, wn, tr);
+        else if constexpr (B > 1) pass_tail<0>(v, t, tw, wn, tr);
     }
 
     template <int M>
     __device__ static __forceinline__ void pass_tail(cpx<T> (&v)[E], int t, const cpx<T>* __restrict__ tw,
-                                                     const cpx<T> (&w)[NW]) {
+                                                     const cpx<T> (&w)[NW], const Tw* tr) {
         // radix-B butterflies j = t + M*TPL on registers v[M + r*G]
 #pragma unroll
-        for (int r = 1; r < B; r++)
-            v[M + r * G] = cmul(v[M + r * G], TWPF ? w[M * (B - 1) + r - 1] : tw_tail(t, M, r, tw));
+        for (int r = 1; r < B; r++) {
+            cpx<T> wr;
+            if constexpr (TWM == 2) wr = dirc(tr->wt[M * (B - 1) + r - 1]);
+            else if constexpr (TWM == 1) wr = w[M * (B - 1) + r - 1];
+            else wr = tw_tail(t, M, r, tw);
+            v[M + r * G] = cmul(v[M + r * G], wr);
+        }
         dft_group<B, DIR, M, G>(v);
-        if constexpr (M + 1 < G) pass_tail<M + 1>(v, t, tw, w);
+        if constexpr (M + 1 < G) pass_tail<M + 1>(v, t, tw, w, tr);
     }
 
     __device__ static __forceinline__ void run(cpx<T> (&v)[E], cpx<T>* lds, int t,
                                                const cpx<T>* __restrict__ tw) {
+        static_assert(TWM != 2, "TWM 2 transforms take their twiddles through run_tw");
         if constexpr (E > 1) {
             cpx<T> w0[NW];
-            pass_full<0>(v, lds, t, tw, w0);
+            pass_full<0>(v, lds, t, tw, w0, nullptr);
+        }
+    }
+    __device__ static __forceinline__ void run_tw(cpx<T> (&v)[E], cpx<T>* lds, int t, const Tw& tr) {
+        static_assert(TWM == 2, "run_tw needs TWM 2");
+        if constexpr (E > 1) {
+            cpx<T> w0[NW];
+            pass_full<0>(v, lds, t, nullptr, w0, &tr);
         }
     }
 };
@@ -286,7 +345,9 @@ __host__ __device__ constexpr int line_e(int n, int emax) { return n < emax ? n 
 __host__ __device__ constexpr int strided_e(int n) { return line_e(n, 16); }
 __host__ __device__ constexpr int strided_tpl(int n) { return n / strided_e(n); }
 __host__ __device__ constexpr int strided_tcols(int n) {
-    return strided_tpl(n) >= 256 ? 1 : 256 / strided_tpl(n);
+    // 512-thread blocks: 32 adjacent columns at n = 256 (512 B per line point
+    // in FP64), wider HBM segments for the strided line accesses
+    return strided_tpl(n) >= 512 ? 1 : 512 / strided_tpl(n);
 }
 template <int N>
 __host__ __device__ constexpr bool line_needs_lds() {

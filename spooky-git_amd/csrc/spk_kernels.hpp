@@ -198,7 +198,6 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     constexpr int RW = zmid_rw<T>(M);
     constexpr int NT = RW * Row::TPL;
     __shared__ cpx<T> lds[RW * Row::LL];
-    __shared__ cpx<T> twl[2 * M];  // tw_nz[0..M-1] | tw_m[0..M-1]
     __shared__ double smax[6][(NT + 63) / 64];
 
     const int tid = threadIdx.x;
@@ -214,17 +213,15 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     cpx<T> nyq[Z::NIN];
 #pragma unroll
     for (int f = 0; f < Z::NIN; f++) Row::load(W + f * fs + row * NZH, active, t, in[f], nyq[f]);
-    // Twiddles are read from LDS: a global twiddle load issued after the stores
-    // of a previous row transform would wait for those stores (vector memory
-    // operations retire in issue order).
-    for (int e = tid; e < M; e += NT) {
-        twl[e] = tw_nz[e];
-        twl[M + e] = tw_m[e];
-    }
-    __syncthreads();
-    const cpx<T> *tz = twl, *tm = twl + M;
+    // Each thread's twiddles are read once, here, and kept in registers for all
+    // NIN + NOUT row transforms: a twiddle load issued after the stores of a
+    // previous row transform would wait for those stores (vector memory
+    // operations retire in issue order), and twiddle reads from LDS collide in
+    // banks.
+    typename Row::RowTw rt;
+    Row::load_tw(rt, t, tw_nz, tw_m);
 #pragma unroll
-    for (int f = 0; f < Z::NIN; f++) Row::c2r_regs(in[f], nyq[f], active, L, t, tz, tm, (T)1);
+    for (int f = 0; f < Z::NIN; f++) Row::c2r_tw(in[f], nyq[f], active, L, t, rt, (T)1);
 
     if constexpr (Z::U) {
         if (pmax) {
@@ -272,7 +269,7 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
             }
             v[i] = cpx<T>{zproduct<PROG>(q, ae, zc), zproduct<PROG>(q, ao, zc)};
         }
-        Row::r2c(v, active, P + q * fs + row * NZH, L, t, tz, tm);
+        Row::r2c_tw(v, active, P + q * fs + row * NZH, L, t, rt);
     }
 }
 

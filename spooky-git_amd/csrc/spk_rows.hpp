@@ -16,16 +16,30 @@
 
 namespace spk {
 
-// TWPF: twiddles are loaded ahead of the LDS exchange that precedes their use
-// (see LineFFT); worth it when they come from LDS or cache with long latency.
-template <class T, int M, int E, bool TWPF = false>
+// REG: the kernel loads a thread's twiddles once (load_tw) and hands them to
+// every transform of the row (c2r_regs / r2c with a RowTw); otherwise each
+// twiddle is read from the tables where it is used.
+template <class T, int M, int E, bool REG = false>
 struct ZRow {
     static constexpr int TPL = M / E;
     static constexpr int LL = lds_len(M + 1);  // LDS elements per row
     // the threads of a row share one wavefront: wavefront-level ordering only
     static constexpr bool WAVE = TPL <= 64;
-    using FFTI = LineFFT<T, M, E, +1, WAVE, TWPF>;
-    using FFTF = LineFFT<T, M, E, -1, WAVE, TWPF>;
+    using FFTI = LineFFT<T, M, E, +1, WAVE, REG ? 2 : 0>;
+    using FFTF = LineFFT<T, M, E, -1, WAVE, REG ? 2 : 0>;
+
+    // a thread's twiddles: the length-M FFT's and W^k = exp(-2 pi i k / NZ) of
+    // its points k = t + i*TPL for the split pre/post-processing
+    struct RowTw {
+        LineTw<T, M, E> f;
+        cpx<T> wz[E];
+    };
+    __device__ static __forceinline__ void load_tw(RowTw &r, int t, const cpx<T> *__restrict__ tw_nz,
+                                                   const cpx<T> *__restrict__ tw_m) {
+        FFTF::load_tw(r.f, t, tw_m);
+#pragma unroll
+        for (int i = 0; i < E; i++) r.wz[i] = tw_nz[t + i * TPL];
+    }
 
     // Global loads of a complex row src[0..M] into registers: v[i] = src[t + i*TPL],
     // nyq = src[M] (thread 0 only).  Kept apart from the transform so a kernel can
@@ -52,6 +66,17 @@ struct ZRow {
     __device__ static __forceinline__ void c2r_regs(cpx<T> (&v)[E], cpx<T> nyq, bool active, cpx<T> *L, int t,
                                                     const cpx<T> *__restrict__ tw_nz,
                                                     const cpx<T> *__restrict__ tw_m, T scale) {
+        static_assert(!REG, "register twiddles: use c2r_tw");
+        c2r_impl(v, nyq, active, L, t, tw_nz, tw_m, nullptr, scale);
+    }
+    __device__ static __forceinline__ void c2r_tw(cpx<T> (&v)[E], cpx<T> nyq, bool active, cpx<T> *L, int t,
+                                                  const RowTw &rt, T scale) {
+        static_assert(REG, "c2r_tw needs REG");
+        c2r_impl(v, nyq, active, L, t, nullptr, nullptr, &rt, scale);
+    }
+    __device__ static __forceinline__ void c2r_impl(cpx<T> (&v)[E], cpx<T> nyq, bool active, cpx<T> *L, int t,
+                                                    const cpx<T> *__restrict__ tw_nz,
+                                                    const cpx<T> *__restrict__ tw_m, const RowTw *rt, T scale) {
         if (active) {
 #pragma unroll
             for (int i = 0; i < E; i++) {
@@ -66,11 +91,6 @@ struct ZRow {
                 L[lpad(M)] = a;
             }
         }
-        cpx<T> wz[E];
-        if constexpr (TWPF) {
-#pragma unroll
-            for (int i = 0; i < E; i++) wz[i] = tw_nz[t + i * TPL];
-        }
         lsync<WAVE>();
 #pragma unroll
         for (int i = 0; i < E; i++) {
@@ -78,11 +98,15 @@ struct ZRow {
             const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
             const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
             const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
-            const cpx<T> wd = cmulc(d, TWPF ? wz[i] : tw_nz[k]);  // W^{-k} d
+            cpx<T> w;
+            if constexpr (REG) w = rt->wz[i];
+            else w = tw_nz[k];
+            const cpx<T> wd = cmulc(d, w);  // W^{-k} d
             v[i] = cpx<T>{s.x - wd.y, s.y + wd.x};  // s + i W^{-k} d
         }
         lsync<WAVE>();
-        FFTI::run(v, L, t, tw_m);
+        if constexpr (REG) FFTI::run_tw(v, L, t, rt->f);
+        else FFTI::run(v, L, t, tw_m);
     }
 
     // z values v (natural layout) -> complex row dst[0..M].  Ends with a barrier
@@ -91,12 +115,21 @@ struct ZRow {
     __device__ static __forceinline__ void r2c(cpx<T> (&v)[E], bool active, cpx<TO> *dst, cpx<T> *L, int t,
                                                const cpx<T> *__restrict__ tw_nz,
                                                const cpx<T> *__restrict__ tw_m) {
-        FFTF::run(v, L, t, tw_m);
-        cpx<T> wz[E];
-        if constexpr (TWPF) {
-#pragma unroll
-            for (int i = 0; i < E; i++) wz[i] = tw_nz[t + i * TPL];
-        }
+        static_assert(!REG, "register twiddles: use r2c_tw");
+        r2c_impl(v, active, dst, L, t, tw_nz, tw_m, nullptr);
+    }
+    template <class TO>
+    __device__ static __forceinline__ void r2c_tw(cpx<T> (&v)[E], bool active, cpx<TO> *dst, cpx<T> *L, int t,
+                                                  const RowTw &rt) {
+        static_assert(REG, "r2c_tw needs REG");
+        r2c_impl(v, active, dst, L, t, nullptr, nullptr, &rt);
+    }
+    template <class TO>
+    __device__ static __forceinline__ void r2c_impl(cpx<T> (&v)[E], bool active, cpx<TO> *dst, cpx<T> *L, int t,
+                                                    const cpx<T> *__restrict__ tw_nz,
+                                                    const cpx<T> *__restrict__ tw_m, const RowTw *rt) {
+        if constexpr (REG) FFTF::run_tw(v, L, t, rt->f);
+        else FFTF::run(v, L, t, tw_m);
 #pragma unroll
         for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
         lsync<WAVE>();
@@ -108,7 +141,10 @@ struct ZRow {
                 const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
                 const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
                 const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
-                dst[k] = cconv<TO>(ev + cmul(od, TWPF ? wz[i] : tw_nz[k]));
+                cpx<T> w;
+                if constexpr (REG) w = rt->wz[i];
+                else w = tw_nz[k];
+                dst[k] = cconv<TO>(ev + cmul(od, w));
             }
             if (t == 0) {
                 const cpx<T> z0 = L[0];
