@@ -84,6 +84,7 @@ struct ZRow {
                 cpx<T> a = cscale(v[i], scale);
                 if (k == 0) a.y = (T)0;
                 L[lpad(k)] = a;
+                v[i] = a;  // X[k] stays in registers; only X[M-k] comes back through LDS
             }
             if (t == 0) {
                 cpx<T> a = cscale(nyq, scale);
@@ -95,7 +96,7 @@ struct ZRow {
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const int k = t + i * TPL;
-            const cpx<T> xk = L[lpad(k)], xm = L[lpad(M - k)];
+            const cpx<T> xk = v[i], xm = L[lpad(M - k)];
             const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
             const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
             cpx<T> w;
@@ -138,7 +139,7 @@ struct ZRow {
 #pragma unroll
             for (int i = 0; i < E; i++) {
                 const int k = t + i * TPL;
-                const cpx<T> zk = L[lpad(k)], zm = L[lpad((M - k) & (M - 1))];
+                const cpx<T> zk = v[i], zm = L[lpad((M - k) & (M - 1))];
                 const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
                 const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
                 cpx<T> w;
