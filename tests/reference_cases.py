@@ -1,0 +1,53 @@
+"""Initial conditions and closed-form solutions of the reference's end-to-end
+tests (tests/*/check-*.py), restated in numpy on the reference's grid
+x_i = -L/2 + i L/N.  Each returns real-space fields {name: [NX, NY, NZ]}."""
+import numpy as np
+
+TWO_PI = 2 * np.pi
+
+
+def grid(n, L=(1.0, 1.0, 1.0)):
+    ax = [-Li / 2 + Li * np.arange(ni) / ni for ni, Li in zip(n, L)]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+def taylor_green_yz(t, nu=1.0 / 200.0, n=(64, 64, 64)):
+    """check-taylor-green.py:14-33 and :118-121: v decays as exp(-2 nu t (2 pi)^2)."""
+    X, Y, Z = grid(n)
+    g = np.exp(-2.0 * nu * t * TWO_PI ** 2)
+    return {"vx": 0 * X, "vy": np.sin(TWO_PI * Y) * np.cos(TWO_PI * Z) * g,
+            "vz": -np.cos(TWO_PI * Y) * np.sin(TWO_PI * Z) * g}
+
+
+def alfven_wave(t, n=(64, 64, 64)):
+    """check-alfven-wave.py:14-38 and :113-123 (its closed form multiplies the
+    real initial fields by exp(-i omega_A t) and keeps the real part)."""
+    X, Y, Z = grid(n)
+    a0 = np.real(1e-5 * np.exp(1j * TWO_PI * Z - 1j * np.pi / 2.0))
+    w = TWO_PI
+    c = np.cos(w * t)
+    zero = 0 * X
+    return {"vx": a0 * c, "vy": zero, "vz": zero, "bx": -a0 * c, "by": zero, "bz": zero + 1.0}
+
+
+MTI_SIGMA = 0.9341413811120219  # check-mti-eigenmode.py:22
+
+
+def mti_eigenmode(t, n=(512, 4, 4)):
+    """check-mti-eigenmode.py:14-44 and :95-108: vz, bz, theta grow as exp(sigma t)."""
+    X, Y, Z = grid(n)
+    Pe, Reeta, B0, N2 = 100.0, 1e5, 1e-4, 0.1
+    kp = TWO_PI * 12.0
+    g = np.exp(MTI_SIGMA * t)
+    vz = -1e-5 * np.sin(kp * X)
+    bz = -1e-5 * np.cos(kp * X) * B0 * kp / (MTI_SIGMA + kp * kp / Reeta)
+    th = 1.0 / (MTI_SIGMA + kp * kp / Pe) * (N2 - kp * kp / Pe / (MTI_SIGMA + kp * kp / Reeta)) * vz
+    zero = 0 * X
+    return {"vx": zero, "vy": zero, "vz": vz * g, "bx": zero + B0, "by": zero, "bz": bz * g, "th": th * g}
+
+
+def l1(a, b, names):
+    """Sum over fields of sum |a - b|, divided by the number of points (the
+    reference's L1 error)."""
+    n = a[names[0]].size
+    return sum(float(np.sum(np.abs(a[k] - b[k]))) for k in names) / n

@@ -36,3 +36,20 @@ def test_abi_version_and_param_layout():
     assert L.spk_params_size() == ctypes.sizeof(spk.SpkParams)
     assert orc.lib().orc_params_size() == ctypes.sizeof(spk.SpkParams)
     assert [f[0] for f in spk.SpkParams._fields_] == [f[0] for f in orc.OrcParams._fields_]
+
+
+def test_snapshot_library_exports_spk_io_h():
+    """libspkio.so (HDF5 snapshots) exports every entry point of include/spk_io.h."""
+    import importlib
+
+    import pytest
+
+    snapshot = importlib.import_module("spooky-git_amd.snapshot")
+    if not os.path.exists(snapshot.LIBPATH):
+        pytest.skip("libspkio.so not built (no HDF5 C library)")
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(HDR, "spk_io.h")).read(), flags=re.S)
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(spkio_\w+)\s*\(", txt, flags=re.M))
+    assert names == {"spkio_last_error", "spkio_write_snapshot", "spkio_read_header", "spkio_read_field",
+                     "spkio_last_snapshot"}
+    L = snapshot.lib()
+    assert all(hasattr(L, n) for n in names)

@@ -1,0 +1,117 @@
+"""The reference's end-to-end tests (tests/*/check-*.py with their own
+spooky.cfg, kept under tests/golden/cfg/) run through the `spooky` driver on
+the GPU exactly as the reference's ctest does: the initial snapshot is written
+(here by the snapshot library instead of h5py), the driver runs with
+`--input-dir <cfg> --output-dir <tmp> -r 0`, and the snapshots it writes are
+read back and compared with the closed form at the reference's tolerance.
+Also: a restart reproduces an uninterrupted run, and timevar.spooky carries
+the configured columns."""
+import importlib
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import reference_cases as rc
+from spkmod import REPO
+
+pytestmark = pytest.mark.gpu
+snapshot = importlib.import_module("spooky-git_amd.snapshot")
+SPOOKY = os.path.join(REPO, "spooky-git_amd", "bin", "spooky")
+CFG = os.path.join(REPO, "tests", "golden", "cfg")
+
+
+def run(cfgdir, outdir, *extra, timeout=600):
+    r = subprocess.run([SPOOKY, "--input-dir", cfgdir, "--output-dir", outdir] + list(extra), capture_output=True,
+                       text=True, timeout=timeout)
+    assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-3000:])
+    return r.stdout
+
+
+def write_ic(outdir, fields):
+    os.makedirs(os.path.join(outdir, "data"), exist_ok=True)
+    snapshot.write_snapshot(snapshot.snapshot_path(os.path.join(outdir, "data"), 0), fields)
+
+
+def read_all(outdir, names, shape):
+    data = os.path.join(outdir, "data")
+    out = []
+    for i in range(snapshot.last_snapshot(data) + 1):
+        hdr, f = snapshot.read_snapshot(snapshot.snapshot_path(data, i), names, shape)
+        out.append((hdr, f))
+    return out
+
+
+def test_taylor_green_vortex(tmp_path):
+    """tests/taylor-green-vortex: L1 < 1e-7 (check-taylor-green.py:34) at every snapshot."""
+    out = str(tmp_path)
+    write_ic(out, rc.taylor_green_yz(0.0))
+    run(os.path.join(CFG, "taylor-green-vortex"), out, "-r", "0")
+    snaps = read_all(out, ["vx", "vy", "vz"], (64, 64, 64))
+    assert len(snaps) == 11  # t = 0, 1, ..., 10 (snapshot_step = 1, t_final = 10)
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.taylor_green_yz(hdr["t_save"]), ["vx", "vy", "vz"])
+        assert err < 1e-7, (hdr, err)
+    assert snaps[-1][0]["t_save"] == pytest.approx(10.0, abs=1e-12)
+    # timevar.spooky: the configured columns, one row per timevar_step; ev is the
+    # kinetic energy 1/4 exp(-4 nu (2 pi)^2 t); hv is not computed here (-1, as
+    # the reference writes for quantities it does not compute)
+    lines = open(os.path.join(out, "data", "timevar.spooky")).read().splitlines()
+    assert lines[1].split() == ["##", "t", "ev", "Kx", "Ky", "Kz", "w2", "hv"]
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines[2:]])
+    assert rows.shape[0] >= 100 and np.all(np.diff(rows[:, 0]) > 0)
+    ev = 0.25 * np.exp(-4 * (1 / 200.0) * rc.TWO_PI ** 2 * rows[:, 0])
+    assert np.max(np.abs(rows[:, 1] - ev) / ev) < 1e-7
+    assert np.all(rows[:, 6] == -1.0)
+
+
+def test_alfven_wave(tmp_path):
+    """tests/alfven-wave: L1 < 1e-7 at t_final = 1 (check-alfven-wave.py:38, one period)."""
+    out = str(tmp_path)
+    names = ["vx", "vy", "vz", "bx", "by", "bz"]
+    write_ic(out, rc.alfven_wave(0.0))
+    run(os.path.join(CFG, "alfven-wave"), out, "-r", "0")
+    snaps = read_all(out, names, (64, 64, 64))
+    hdr, f = snaps[-1]
+    assert hdr["t_save"] == pytest.approx(1.0, abs=1e-12)
+    assert rc.l1(f, rc.alfven_wave(hdr["t_save"]), names) < 1e-7
+
+
+def test_mti_eigenmode(tmp_path):
+    """tests/mti-eigenmode (512x4x4, anisotropic conduction): L1 < 1e-7
+    (check-mti-eigenmode.py:47) at every snapshot up to t_final = 2."""
+    out = str(tmp_path)
+    names = ["vx", "vy", "vz", "bx", "by", "bz", "th"]
+    write_ic(out, rc.mti_eigenmode(0.0))
+    run(os.path.join(CFG, "mti-eigenmode"), out, "-r", "0")
+    snaps = read_all(out, names, (512, 4, 4))
+    assert snaps[-1][0]["t_save"] == pytest.approx(2.0, abs=1e-12)
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.mti_eigenmode(hdr["t_save"]), names)
+        assert err < 1e-7, (hdr, err)
+
+
+def test_restart_reproduces_uninterrupted_run(tmp_path):
+    """Snapshots 0-2 of one run; a second run restarted from snapshot 1 (-r 1)
+    writes the same snapshot 2 (ReadDataFile, hdf5_io.cu:110-252)."""
+    cfg = tmp_path / "cfg"
+    cfg.mkdir()
+    text = open(os.path.join(CFG, "taylor-green-vortex", "spooky.cfg")).read()
+    text = text.replace("t_final = 10.0;", "t_final = 2.0;")
+    assert "t_final = 2.0;" in text
+    (cfg / "spooky.cfg").write_text(text)
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    write_ic(a, rc.taylor_green_yz(0.0))
+    run(str(cfg), a, "-r", "0")
+    os.makedirs(os.path.join(b, "data"))
+    for i in (0, 1):
+        shutil.copy(snapshot.snapshot_path(os.path.join(a, "data"), i), os.path.join(b, "data"))
+    run(str(cfg), b, "-r", "1")
+    names, shape = ["vx", "vy", "vz"], (64, 64, 64)
+    ha, fa = snapshot.read_snapshot(snapshot.snapshot_path(os.path.join(a, "data"), 2), names, shape)
+    hb, fb = snapshot.read_snapshot(snapshot.snapshot_path(os.path.join(b, "data"), 2), names, shape)
+    assert ha["step"] == hb["step"] and ha["t_save"] == pytest.approx(hb["t_save"], rel=1e-13)
+    for k in names:
+        assert np.max(np.abs(fa[k] - fb[k])) < 1e-12
