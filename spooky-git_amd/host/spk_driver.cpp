@@ -224,9 +224,8 @@ struct Run {
         std::fclose(f);
     }
 
-    // CheckOutput (check_output.cu:9-94); snapshot_due comes from the device
-    // snapshot clock, which the main loop advances exactly as CheckSnapshot
-    void check_output(bool snapshot_due) {
+    // CheckTimeSeries (check_output.cu:15-51)
+    void check_time_series() {
         const double t = tm[0];
         if ((t - t_lastvar) >= rp.toutput_time || t == 0.0) {
             CALL(spk_clean_divergence(sol));
@@ -235,11 +234,20 @@ struct Run {
             if (t != 0.0) t_lastvar += rp.toutput_time;
             if (!rp.timevar_vars.empty()) timevar_row();
         }
-        if (snapshot_due || t == 0.0) {
+    }
+    // CheckSnapshot (check_output.cu:53-94); due comes from the device snapshot
+    // clock, which the main loop advances exactly as CheckSnapshot does
+    void check_snapshot(bool due) {
+        const double t = tm[0];
+        if (due || t == 0.0) {
             CALL(spk_clean_divergence(sol));
             std::printf("Saving data snap at t= %.6e \t and step n. %d \n", t, (int)tm[7]);
             write_snapshot();
         }
+    }
+    void check_output(bool snapshot_due) {
+        check_time_series();
+        check_snapshot(snapshot_due);
     }
 };
 
@@ -295,7 +303,10 @@ int main(int argc, char **argv) {
     if (restart) R.read_snapshot(cli.restart_num == -2 ? -1 : cli.restart_num);
     else R.initial_state();
     R.read_time();
-    if (!restart) R.check_output(false);  // initialDataDump (supervisor.cu:215-249)
+    // initialDataDump (supervisor.cu:215-239): all outputs for a fresh start,
+    // the time series only when restarting from snapshot 0
+    if (!restart) R.check_output(false);
+    else if (cli.restart_num == 0) R.check_time_series();
 
     // executeMainLoop (supervisor.cu:169-213)
     const auto t0 = std::chrono::steady_clock::now();
