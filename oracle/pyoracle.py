@@ -240,7 +240,8 @@ class Sim:
         return out[:nb].copy()
 
 
-def rkl2_coeffs(dt, dt_sts, cap=1024):
-    coef = np.zeros(4 * cap)
-    s = lib().orc_rkl2_coeffs(dt, dt_sts, _ptr(coef), cap)
+def rkl2_coeffs(dt, dt_sts):
+    s = lib().orc_rkl2_coeffs(dt, dt_sts, None, 0)
+    coef = np.zeros(4 * (s + 1))
+    lib().orc_rkl2_coeffs(dt, dt_sts, _ptr(coef), s)
     return s, coef[: 4 * s].reshape(s, 4)

@@ -913,8 +913,10 @@ void orc_rkl2_cycle(orc_sim *s) {
     int ns = s->nsts;
     if (ns == 0) return;
     double dt_current = s->dt;
-    double coef[4 * 512];
-    int s_RKL = orc_rkl2_coeffs(dt_current, s->dt_sts, coef, 512);
+    /* the stage count follows dt / dt_sts without bound (rkl.cu:466-470) */
+    int s_RKL = orc_rkl2_coeffs(dt_current, s->dt_sts, NULL, 0);
+    double *coef = (double *)malloc(sizeof(double) * 4 * (size_t)(s_RKL + 1));
+    orc_rkl2_coeffs(dt_current, s->dt_sts, coef, s_RKL);
     memset(s->dU0, 0, sizeof(double) * 2 * N * ns);
     memset(s->dU, 0, sizeof(double) * 2 * N * ns);
     parabolic_terms_sts(s, s->dU);
@@ -930,13 +932,8 @@ void orc_rkl2_cycle(orc_sim *s) {
         axpy(N, Uc1, dU0, U, 1.0, mu_tilde_1 * dt_current);
     }
     for (int stage = 2; stage <= s_RKL; stage++) {
-        double mu_j, nu_j, mu_tilde_j, gamma_j;
-        if (stage <= 512) {
-            const double *c = coef + 4 * (stage - 1);
-            mu_j = c[0]; nu_j = c[1]; mu_tilde_j = c[2]; gamma_j = c[3];
-        } else {
-            return; /* more than 512 stages is outside any supported configuration */
-        }
+        const double *c = coef + 4 * (stage - 1);
+        const double mu_j = c[0], nu_j = c[1], mu_tilde_j = c[2], gamma_j = c[3];
         memset(s->dU, 0, sizeof(double) * 2 * N * ns);
         parabolic_terms_sts(s, s->dU);
         for (int nv = 0; nv < ns; nv++) {
@@ -955,6 +952,7 @@ void orc_rkl2_cycle(orc_sim *s) {
             }
         }
     }
+    free(coef);
 }
 
 /* hydro_mhd_advance.cu:18-75 */

@@ -51,3 +51,16 @@ def l1(a, b, names):
     reference's L1 error)."""
     n = a[names[0]].size
     return sum(float(np.sum(np.abs(a[k] - b[k]))) for k in names) / n
+
+
+def advected_mhd_vortex(t, n=(512, 512, 2)):
+    """check-advected-mhd-vortex.py:14-46: a Gaussian MHD vortex (sigma = 200)
+    carried by the uniform flow (0.05, 0.05); ideal closed form (the run has
+    nu = eta = 1e-4, hence the reference's tolerance of 1e-4)."""
+    X, Y, Z = grid(n)
+    u0 = v0 = 0.05
+    sigma = 200.0
+    Xt, Yt = X - u0 * t, Y - v0 * t
+    g = np.exp((1.0 - sigma * (Xt ** 2 + Yt ** 2)) / 2.0) / TWO_PI
+    zero = 0 * X
+    return {"vx": v0 - g * Yt, "vy": u0 + g * Xt, "vz": zero, "bx": -g * Yt, "by": g * Xt, "bz": zero}

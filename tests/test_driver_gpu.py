@@ -7,6 +7,7 @@ read back and compared with the closed form at the reference's tolerance.
 Also: a restart reproduces an uninterrupted run, and timevar.spooky carries
 the configured columns."""
 import importlib
+import json
 import os
 import shutil
 import subprocess
@@ -115,3 +116,54 @@ def test_restart_reproduces_uninterrupted_run(tmp_path):
     assert ha["step"] == hb["step"] and ha["t_save"] == pytest.approx(hb["t_save"], rel=1e-13)
     for k in names:
         assert np.max(np.abs(fa[k] - fb[k])) < 1e-12
+
+
+@pytest.mark.parametrize("nx", [32, 128, 512])
+def test_heat_diffusion_rkl2(tmp_path, nx):
+    """tests/heat-diffusion-sts, rkl2 column (check-heat-diffusion-sts.py:60-122):
+    a Gaussian (sigma 0.1, lx = 4) diffusing with nu_th = 1, run with
+    --ngrid nx 2 2 -r 0.  With this configuration the reference takes one
+    dt = t_final step (its sts cap sits in the incompressible branch of
+    compute_dt.cu:272-283, and the t_final clamp pre-empts the snapshot clamp,
+    :288-293), i.e. one RKL2 cycle of many stages.  The last snapshot equals
+    the FP64 oracle's state to 1e-12."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc
+
+    cfg = os.path.join(CFG, "heat-diffusion-sts-rkl2")
+    out = str(tmp_path)
+    lx, sig = 4.0, 0.1
+    X = np.broadcast_to((-0.5 * lx + lx * np.arange(nx) / nx)[:, None, None], (nx, 2, 2))
+    T0 = 1 / np.sqrt(2 * np.pi * sig ** 2) * np.exp(-0.5 * X ** 2 / sig ** 2)
+    write_ic(out, {"th": T0})
+    run(cfg, out, "--ngrid", str(nx), "2", "2", "-r", "0")
+    hdr, f = read_all(out, ["th"], (nx, 2, 2))[-1]
+    assert hdr["t_save"] == pytest.approx(0.5, abs=1e-14)
+    p = json.loads(subprocess.run([SPOOKY, "--input-dir", cfg, "--print-params"], capture_output=True,
+                                  text=True).stdout.strip().splitlines()[-1])
+    q = {k: p[k] for k in orc.DEFAULTS if k in p}
+    q.update(nx=nx, ny=2, nz=2)
+    sim = orc.Sim(**q)
+    sim.set_fields(np.fft.rfftn(T0)[None])
+    while sim.time()["t"] < 0.5 - 1e-14:
+        sim.main_loop(1)
+    assert sim.time()["step"] == hdr["step"]
+    ref = np.fft.irfftn(sim.fields[0], s=(nx, 2, 2), axes=(0, 1, 2))
+    assert np.max(np.abs(f["th"] - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_advected_mhd_vortex(tmp_path):
+    """tests/advected-mhd-vortex (512x512x2): L1 < 1e-4 (check-advected-mhd-vortex.py:47)
+    at every snapshot."""
+    out = str(tmp_path)
+    names = ["vx", "vy", "vz", "bx", "by", "bz"]
+    write_ic(out, rc.advected_mhd_vortex(0.0))
+    run(os.path.join(CFG, "advected-mhd-vortex"), out, "-r", "0")
+    snaps = read_all(out, names, (512, 512, 2))
+    # snapshots every 0.2 up to t = 1.0; the run ends at t_final = 1.1 between two
+    # snapshot times, so no snapshot is written there (CheckSnapshot, check_output.cu:62)
+    assert len(snaps) == 6 and snaps[-1][0]["t_save"] == pytest.approx(1.0, abs=1e-12)
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.advected_mhd_vortex(hdr["t_save"]), names)
+        assert err < 1e-4, (hdr, err)

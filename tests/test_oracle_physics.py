@@ -211,7 +211,8 @@ def _rkl2_R(x, coef):
 
 
 def test_rkl2_coefficients():
-    for r in (1.0, 7.3, 40.0, 100.0):
+    # r = 1.6e5 needs about 800 stages (the 512^2 x 2 x 2 heat-diffusion-sts run)
+    for r in (1.0, 7.3, 40.0, 100.0, 1.6e5):
         s, coef = orc.rkl2_coeffs(r, 1.0)
         # stage count: 4 r / dt_exp = s^2 + s - 2 solved for s (rkl.cu:473-477)
         assert s == 1 + int(4 * (1 + 2 * r) / (1 + math.sqrt(9 + 16 * r)))
