@@ -48,9 +48,10 @@ def parse():
     ap.add_argument("--mti-steps", type=int, default=10)
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the Taylor-Green 64^3 and MTI legs")
-    ap.add_argument("--mode", choices=["full", "passes", "kernels"], default="full",
+    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti"], default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
-                         "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs)")
+                         "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs); "
+                         "mti: only the MTI leg (--mti-steps steps)")
     return ap.parse_args()
 
 
@@ -307,6 +308,9 @@ def main():
         passes = time_fft_passes(spk, torch, ctx, "f64", n, reps=5)
         copy = copy_gbs(torch, n * n * (n // 2 + 1) * 16 * 8)
         print(json.dumps({"fft_pass": passes, "copy_gbs": copy, "copy_bytes": n * n * (n // 2 + 1) * 16 * 8}))
+        return
+    if a.mode == "mti":
+        print(json.dumps(mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu)))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
     if a.mode == "kernels":
