@@ -94,11 +94,15 @@ __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_inv_x(
 // fused z kernel: C2R of the input rows, real-space products, R2C of the
 // products.  Real-space fields never reach HBM.
 // ------------------------------------------------------------------------
-enum : int { ZP_U = 1, ZP_B = 2, ZP_TH = 4, ZP_G = 8 };
+// ZP_BR: the B rows come in real space (z values of a padded real row, read as
+// NZ/2 complex pairs) from a second buffer instead of through a C2R -- B does
+// not change during an RKL2 cycle, so its real-space rows are made once per cycle
+enum : int { ZP_U = 1, ZP_B = 2, ZP_TH = 4, ZP_G = 8, ZP_BR = 16 };
 
 template <int PROG>
 struct ZProg {
     static constexpr bool U = PROG & ZP_U, B = PROG & ZP_B, TH = PROG & ZP_TH, G = PROG & ZP_G;
+    static constexpr bool BR = (PROG & ZP_BR) && B;
     // inputs: u(3) | B(3) | theta | grad theta(3)
     static constexpr int IU = 0, IB = IU + (U ? 3 : 0), ITH = IB + (B ? 3 : 0), IG = ITH + (TH ? 1 : 0);
     static constexpr int NIN = IG + (G ? 3 : 0);
@@ -185,8 +189,8 @@ __host__ __device__ constexpr int zmid_rw(int m) {
 }
 
 template <class T, int NZ, int PROG>
-__global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<T> *__restrict__ P,
-                                              long long nrows, long long fs,
+__global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, const cpx<T> *__restrict__ Wr,
+                                              cpx<T> *__restrict__ P, long long nrows, long long fs,
                                               const cpx<T> *__restrict__ tw_nz,
                                               const cpx<T> *__restrict__ tw_m, ZConst zc,
                                               double *__restrict__ pmax) {
@@ -212,7 +216,15 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     cpx<T> in[Z::NIN][E];
     cpx<T> nyq[Z::NIN];
 #pragma unroll
-    for (int f = 0; f < Z::NIN; f++) Row::load(W + f * fs + row * NZH, active, t, in[f], nyq[f]);
+    for (int f = 0; f < Z::NIN; f++) {
+        if (Z::BR && f >= Z::IB && f < Z::IB + 3) {
+            const cpx<T> *r = Wr + (f - Z::IB) * fs + row * NZH;
+#pragma unroll
+            for (int i = 0; i < E; i++) in[f][i] = active ? r[t + i * Row::TPL] : cpx<T>{(T)0, (T)0};
+        } else {
+            Row::load(W + f * fs + row * NZH, active, t, in[f], nyq[f]);
+        }
+    }
     // Each thread's twiddles are read once, here, and kept in registers for all
     // NIN + NOUT row transforms: a twiddle load issued after the stores of a
     // previous row transform would wait for those stores (vector memory
@@ -221,7 +233,8 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cpx<
     typename Row::RowTw rt;
     Row::load_tw(rt, t, tw_nz, tw_m);
 #pragma unroll
-    for (int f = 0; f < Z::NIN; f++) Row::c2r_tw(in[f], nyq[f], active, L, t, rt, (T)1);
+    for (int f = 0; f < Z::NIN; f++)
+        if (!(Z::BR && f >= Z::IB && f < Z::IB + 3)) Row::c2r_tw(in[f], nyq[f], active, L, t, rt, (T)1);
 
     if constexpr (Z::U) {
         if (pmax) {

@@ -395,6 +395,9 @@ struct spk_solver_s {
     cpx<double> *U, *S, *dU;
     double *realbuf;
     void *W, *P, *W2, *P2;
+    void *Breal;        // real-space B rows (padded, bulk precision), made once per RKL2 cycle
+    int rkl_cache_b;    // B is not an sts variable: its real-space rows stay valid for a whole cycle
+    InvProg inv_rkl_b, inv_rkl_g;
     cpx<double> *rUc0, *rUc1, *rdU0;
     TimeState *ts;
     double *pmax, *part, *res, *spec;
@@ -461,19 +464,23 @@ int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax,
     const cpx<T> *tz = twp<T>(s->twb, s->twb.off_z), *tm = twp<T>(s->twb, s->twb.off_z + s->g.nz);
     ZConst zc{1.0 / s->p.reynolds_ani, s->p.OmegaT2, s->p.strat_direction};
     const cpx<T> *w = (const cpx<T> *)W;
+    const cpx<T> *wr = (const cpx<T> *)s->Breal;
     cpx<T> *pp = (cpx<T> *)P;
     hipStream_t st = s->ctx->stream;
     switch (prog) {
-        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
-        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
-        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
         case ZP_U | ZP_B | ZP_TH:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
             break;
         case ZP_U | ZP_B | ZP_TH | ZP_G:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
             break;
-        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_B | ZP_G | ZP_BR:
+            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            break;
         default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
     }
     SPK_LAUNCH_CHECK();
@@ -786,14 +793,46 @@ int rkl2_front(spk_solver s) {
     return SPK_OK;
 }
 
+// Real-space B rows for a whole RKL2 cycle (B is not an sts variable there):
+// inverse x, inverse y and z C2R of the three B fields into Breal.
+int rkl2_b_real(spk_solver s) {
+    const Grid3 &g = s->g;
+    const double ks = s->tremap * s->p.shear;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_b, 3, s->W2, ks));
+    else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_b, 3, s->W2, ks));
+    PassArgs a{s->W2, s->W2, 3, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    PassArgs z{s->W2, s->Breal, 3, g.ntc, g.ntc, 1.0};
+    return launch_pass_z_c2r(s->ctx, g, s->twb, s->bulk, s->bulk, z);
+}
+
+// As rkl2_front with B taken from Breal: only i k theta is transformed per stage.
+int rkl2_front_cached(spk_solver s) {
+    const Grid3 &g = s->g;
+    const double ks = s->tremap * s->p.shear;
+    const size_t fb = (size_t)g.ntc * esize(s->bulk);
+    void *wg = (char *)s->W2 + 3 * fb;  // the i k theta slots of the fused z program
+    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_g, 3, wg, ks));
+    else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_g, 3, wg, ks));
+    PassArgs a{wg, wg, 3, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
+    else SPK_TRY(launch_zmid<float>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
+    PassArgs b{s->P2, s->P2, 3, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b));
+    SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b));
+    return SPK_OK;
+}
+
 int rkl2_cycle(spk_solver s) {
     if (s->nsts == 0) return SPK_OK;
     TimeState h;
     SPK_TRY(read_time(s, &h));
     std::vector<double> coef;
     const int sr = rkl2_coeffs(h.dt, h.dt_sts, coef);
+    if (s->rkl_aniso && s->rkl_cache_b) SPK_TRY(rkl2_b_real(s));
     for (int stage = 1; stage <= sr; stage++) {
-        if (s->rkl_aniso) SPK_TRY(rkl2_front(s));
+        if (s->rkl_aniso) SPK_TRY(s->rkl_cache_b ? rkl2_front_cached(s) : rkl2_front(s));
         const double *c = &coef[4 * (size_t)(stage - 1)];
         const double mu = c[0], nu = c[1], mut = c[2], gam = c[3];
         const double dtmut = stage == 1 ? mut * h.dt : h.dt * mut;
@@ -901,7 +940,7 @@ int step_fused(spk_solver s, bool w_valid) {
 }
 
 void free_all(spk_solver s) {
-    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->rUc0, s->rUc1, s->rdU0,
+    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->Breal, s->rUc0, s->rUc1, s->rdU0,
                     s->ts, s->pmax, s->part, s->res, s->spec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1017,6 +1056,10 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     memset(&s->inv_rkl, 0, sizeof(s->inv_rkl));
     for (int q = 0; q < 3; q++) s->inv_rkl.s[q] = InvSlot{s->BX + q, -1};
     for (int q = 0; q < 3; q++) s->inv_rkl.s[3 + q] = InvSlot{s->TH, q};
+    memset(&s->inv_rkl_b, 0, sizeof(s->inv_rkl_b));
+    memset(&s->inv_rkl_g, 0, sizeof(s->inv_rkl_g));
+    for (int q = 0; q < 3; q++) s->inv_rkl_b.s[q] = InvSlot{s->BX + q, -1};
+    for (int q = 0; q < 3; q++) s->inv_rkl_g.s[q] = InvSlot{s->TH, q};
     memset(&s->rkl, 0, sizeof(s->rkl));
     s->rkl.ntc = s->g.ntc;
     s->rkl.nsts = s->nsts;
@@ -1040,6 +1083,8 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     if (s->rkl_aniso) {
         SPK_TRY(dalloc(&s->W2, fb * 6));
         SPK_TRY(dalloc(&s->P2, fb * 3));
+        s->rkl_cache_b = s->sts_pos[s->BX] < 0 && s->sts_pos[s->BX + 1] < 0 && s->sts_pos[s->BX + 2] < 0;
+        if (s->rkl_cache_b) SPK_TRY(dalloc(&s->Breal, fb * 3));
     }
     if (s->nsts > 0) {
         SPK_TRY(dalloc(&s->rUc0, f64 * s->nsts));
