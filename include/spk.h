@@ -154,6 +154,16 @@ int spk_correlation(spk_solver s, int fa, int fb, double *out);  /* twoFieldCorr
 /* Spectrum1d (cuda_kernels_generic.cu:441-471) + computeSpectrum1d
  * (spooky_output_spectrum.cu:424-444); out[nbins]; bins beyond nbins are dropped */
 int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins);
+/* One quantity of the timevar file by the reference's name, as WriteTimevarOutput
+ * computes it (spooky_output_timevar.cu:48-204, spooky_output_functions.cu:23-395):
+ * ev em et Kx Ky Kz Mx My Mz vxvy vxvz vyvz bxby bxbz bybz thvx thvz w2 j2 hc hv
+ * hm thetaB bz2 phiB pv dissgradT fluxbbgradT.  Quantities from real fields read
+ * the real buffer, which must hold the current state (spk_complex2real), as the
+ * reference fills d_all_buffer_r before writing a row.  SPK_ERR_ARG for a name
+ * that is not a quantity of this configuration ("t" is the host's clock).
+ * Differences: computeAnisoDissipation accumulates into scratch it does not
+ * clear (DivergenceMask flag 1); here the divergence is written, not added. */
+int spk_timevar(spk_solver s, const char *name, double *out);
 
 #ifdef __cplusplus
 }

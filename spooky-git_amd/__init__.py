@@ -104,6 +104,7 @@ SIGNATURES = {
     "spk_enstrophy": [_VP, _I, _VP],
     "spk_correlation": [_VP, _I, _I, _VP],
     "spk_spectrum": [_VP, _I, _I, _VP, _I, _VP],
+    "spk_timevar": [_VP, ctypes.c_char_p, _VP],
 }
 
 
@@ -356,6 +357,13 @@ class Solver:
     def correlation(self, fa, fb):
         o = ctypes.c_double()
         check(lib().spk_correlation(self.h, fa, fb, ctypes.byref(o)))
+        return o.value
+
+    def timevar(self, name):
+        """One timevar quantity by the reference's name (ev, Kx, hv, thetaB, dissgradT, ...);
+        the real buffer must hold the current state (complex2real)."""
+        o = ctypes.c_double()
+        check(lib().spk_timevar(self.h, name.encode(), ctypes.byref(o)))
         return o.value
 
     def spectrum(self, fa, fb, cap=4096):

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace spk {
@@ -273,7 +274,8 @@ __device__ __forceinline__ double block_sum(double v, double *sm) {
     return s;
 }
 
-// kind 0: energy weight sum |v|^2 (computeEnergy); kind 1: enstrophy (Curl + computeEnergy)
+// kind 0: energy weight sum |v|^2 (computeEnergy); kind 1: enstrophy (Curl +
+// computeEnergy); kind 2: energy of the gradient (computeDissipation)
 __global__ __launch_bounds__(256) void k_spec_sum(const cpx<double> *__restrict__ U, long long ntc, KGeom kg,
                                                   int kind, double *__restrict__ part) {
     __shared__ double sm[4];
@@ -286,6 +288,13 @@ __global__ __launch_bounds__(256) void k_spec_sum(const cpx<double> *__restrict_
         if (kind == 0) {
             const cpx<double> a = U[idx];
             acc += w * (a.x * a.x + a.y * a.y);
+        } else if (kind == 2) {
+            // sum of the energies of i k_n a (Gradient + computeEnergy, computeDissipation)
+            const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+            double kx, ky, kz, m;
+            kvec_at(kg, i, j, k, kx, ky, kz, m);
+            const cpx<double> a = U[idx];
+            acc += w * (kx * kx + ky * ky + kz * kz) * (a.x * a.x + a.y * a.y);
         } else {
             const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
             double kx, ky, kz, m;
@@ -359,6 +368,104 @@ __global__ void k_zero_d(double *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
+// ---- timevar diagnostics (spooky_output_functions.cu) ----
+
+// spectral 3-vector from the state: kind 0 Helicity (i k x V / k^2, k^2 = 0 -> 1,
+// cuda_kernels.cu Helicity), 1 Curl (i k x V), 2 Gradient of the scalar V[0] (i k V)
+__global__ __launch_bounds__(256) void k_diag_vec(const cpx<double> *__restrict__ V, cpx<double> *__restrict__ Z,
+                                                  long long ntc, KGeom kg, int kind) {
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+        double kx, ky, kz, m;
+        kvec_at(kg, i, j, k, kx, ky, kz, m);
+        cpx<double> z0, z1, z2;
+        if (kind == 2) {
+            const cpx<double> a = V[idx];
+            z0 = {-kx * a.y, kx * a.x};
+            z1 = {-ky * a.y, ky * a.x};
+            z2 = {-kz * a.y, kz * a.x};
+        } else {
+            const cpx<double> V0 = V[idx], V1 = V[ntc + idx], V2 = V[2 * ntc + idx];
+            // i (k x V)
+            const cpx<double> c0 = {ky * V2.x - kz * V1.x, ky * V2.y - kz * V1.y};
+            const cpx<double> c1 = {kz * V0.x - kx * V2.x, kz * V0.y - kx * V2.y};
+            const cpx<double> c2 = {kx * V1.x - ky * V0.x, kx * V1.y - ky * V0.y};
+            double f = 1.0;
+            if (kind == 0) {
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                f = k2 > 0.0 ? 1.0 / k2 : 1.0;
+            }
+            z0 = {-f * c0.y, f * c0.x};
+            z1 = {-f * c1.y, f * c1.x};
+            z2 = {-f * c2.y, f * c2.x};
+        }
+        Z[idx] = z0;
+        Z[ntc + idx] = z1;
+        Z[2 * ntc + idx] = z2;
+    }
+}
+
+// real-space 3-vector from B (padded rows, every value including padding):
+// kind 0 Computebbstrat (B_i B_s / B^2, cuda_kernels.cu:385-398); kind 1
+// ComputeBGradTheta + ComputeAnisotropicHeatFlux with chi = 1, OmegaT2 = 0
+// (B_i (B . G) / B^2, cuda_kernels.cu:360-383); G: real grad theta
+__global__ __launch_bounds__(256) void k_diag_bvec(const double *__restrict__ B, const double *__restrict__ G,
+                                                   double *__restrict__ Z, long long n, int sd, int kind) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double b0 = B[i], b1 = B[n + i], b2 = B[2 * n + i];
+        const double B2 = b0 * b0 + b1 * b1 + b2 * b2;
+        double c;
+        if (kind == 0) c = (sd == 0 ? b0 : (sd == 1 ? b1 : b2)) / B2;
+        else c = (b0 * G[i] + b1 * G[n + i] + b2 * G[2 * n + i]) / B2;
+        Z[i] = b0 * c;
+        Z[n + i] = b1 * c;
+        Z[2 * n + i] = b2 * c;
+    }
+}
+
+// DivergenceMask (cuda_kernels_generic.cu:315-334): i mask (k . X)
+__global__ __launch_bounds__(256) void k_diag_div(const cpx<double> *__restrict__ X, cpx<double> *__restrict__ Z,
+                                                  long long ntc, KGeom kg) {
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+        double kx, ky, kz, m;
+        kvec_at(kg, i, j, k, kx, ky, kz, m);
+        const cpx<double> d = kdot3(kx, X[idx], ky, X[ntc + idx], kz, X[2 * ntc + idx]);
+        Z[idx] = cpx<double>{-m * d.y, m * d.x};
+    }
+}
+
+// physical-point sums of a function of real B: kind 0 |b_z| (averagebz), 1 b_z^2
+// (averagebz2) with b = B / (|B| + 1e-16) (bUnitvector); 2 acosf(|B_x| / (|B_xy| +
+// 1e-16)) (averagephiB, AngleHorizPlane: single-precision acos as there)
+__global__ __launch_bounds__(256) void k_diag_bsum(const double *__restrict__ B, long long fs, long long nrows, int nz,
+                                                   int nzp, int kind, double *__restrict__ part) {
+    __shared__ double sm[4];
+    double acc = 0.0;
+    const long long n = nrows * nz;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long r = p / nz;
+        const long long o = r * nzp + (p - r * nz);
+        const double bx = B[o], by = B[fs + o], bz = B[2 * fs + o];
+        if (kind == 2) {
+            const double nrm = 1e-16 + sqrt(bx * bx + by * by);
+            acc += (double)acosf((float)(fabs(bx) / nrm));
+        } else {
+            const double nrm = 1e-16 + sqrt(bx * bx + by * by + bz * bz);
+            const double u = bz / nrm;
+            acc += kind == 0 ? fabs(u) : u * u;
+        }
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 }  // namespace spk
 
 namespace spk {
@@ -396,6 +503,7 @@ struct spk_solver_s {
     double *realbuf;
     void *W, *P, *W2, *P2;
     void *Breal;        // real-space B rows (padded, bulk precision), made once per RKL2 cycle
+    cpx<double> *diag;  // 6 FP64 fields of scratch for the timevar diagnostics, allocated at first use
     int rkl_cache_b;    // B is not an sts variable: its real-space rows stay valid for a whole cycle
     InvProg inv_rkl_b, inv_rkl_g;
     cpx<double> *rUc0, *rUc1, *rdU0;
@@ -940,7 +1048,7 @@ int step_fused(spk_solver s, bool w_valid) {
 }
 
 void free_all(spk_solver s) {
-    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->Breal, s->rUc0, s->rUc1, s->rdU0,
+    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->Breal, s->diag, s->rUc0, s->rUc1, s->rdU0,
                     s->ts, s->pmax, s->part, s->res, s->spec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1318,6 +1426,179 @@ extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, 
     SPK_HIP(hipStreamSynchronize(s->ctx->stream));
     for (int i = 0; i < nb && i < cap; i++) out[i] = h[i];
     return SPK_OK;
+}
+
+namespace {
+
+// FP64 3D inverse transform of n complex fields (1/N folded in) into padded real rows
+int diag_c2r(spk_solver s, const void *in, void *out, int n) {
+    const Grid3 &g = s->g;
+    PassArgs a{in, out, n, g.ntc, g.ntc, 1.0 / ((double)g.nx * g.ny * g.nz)};
+    SPK_TRY(launch_pass_x(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, a));
+    PassArgs b{out, out, n, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, b));
+    return launch_pass_z_c2r(s->ctx, g, s->tw64, SPK_F64, SPK_F64, b);
+}
+
+// FP64 3D forward transform of n padded real fields into complex fields
+int diag_r2c(spk_solver s, const void *in, void *out, int n) {
+    const Grid3 &g = s->g;
+    PassArgs a{in, out, n, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_z_r2c(s->ctx, g, s->tw64, SPK_F64, SPK_F64, a));
+    PassArgs b{out, out, n, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->tw64, -1, SPK_F64, SPK_F64, b));
+    return launch_pass_x(s->ctx, g, s->tw64, -1, SPK_F64, SPK_F64, b);
+}
+
+// twoFieldCorrelation of padded real fields: sum over physical points / N, added to *acc
+int diag_dot(spk_solver s, const double *a, const double *b, double *acc) {
+    const long long rows = (long long)s->g.nx * s->g.ny;
+    const unsigned nb = red_grid(rows * s->g.nz);
+    k_real_dot<<<nb, 256, 0, s->ctx->stream>>>(a, b, rows, s->g.nz, 2 * s->g.nzh, s->part);
+    SPK_LAUNCH_CHECK();
+    double v;
+    SPK_TRY(reduce_final(s, (int)nb, 1.0 / ((double)s->g.nx * s->g.ny * s->g.nz), &v));
+    *acc += v;
+    return SPK_OK;
+}
+
+int diag_spec_sum(spk_solver s, const cpx<double> *u, int kind, const KGeom &kg, double *out) {
+    const unsigned nb = red_grid(s->g.ntc);
+    k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(u, s->g.ntc, kg, kind, s->part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)s->g.nx * s->g.ny * s->g.nz;
+    return reduce_final(s, (int)nb, 1.0 / (nt * nt), out);
+}
+
+}  // namespace
+
+// WriteTimevarOutput (spooky_output_timevar.cu:48-204), one quantity by name
+extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(name && out, SPK_ERR_ARG, "spk_timevar: null argument");
+    const std::string v(name);
+    const spk_params &p = s->p;
+    const Grid3 &g = s->g;
+    const long long fr = 2 * g.ntc;  // doubles per padded real field
+    const KGeom kg = kgeom(s, s->tremap * p.shear);
+    const double *R = s->realbuf;
+    const int VX = s->VX, BX = s->BX, TH = s->TH;
+    auto energy = [&](int f, double *o) { return diag_spec_sum(s, s->U + (long long)f * g.ntc, 0, kg, o); };
+    auto corr = [&](int a, int b, double *o) {
+        *o = 0.0;
+        return diag_dot(s, R + a * fr, R + b * fr, o);
+    };
+    auto need_diag = [&]() -> int {
+        if (!s->diag) {
+            hipError_t e = hipMalloc((void **)&s->diag, (size_t)g.ntc * 16 * 6);
+            if (e != hipSuccess) {
+                s->diag = nullptr;
+                set_error("hipMalloc(diagnostic scratch): %s", hipGetErrorString(e));
+                return SPK_ERR_ALLOC;
+            }
+        }
+        return SPK_OK;
+    };
+    const unsigned gb = red_grid(g.ntc);
+    hipStream_t st = s->ctx->stream;
+    double a = 0.0, b = 0.0, c = 0.0;
+    const bool inc = p.incompressible, mhd = p.mhd, th = p.boussinesq || p.heat_equation;
+    if (inc && (v == "ev" || v == "Kx" || v == "Ky" || v == "Kz")) {
+        if (v != "ev") return energy(VX + (v[1] - 'x'), out);
+        SPK_TRY(energy(VX, &a)); SPK_TRY(energy(VX + 1, &b)); SPK_TRY(energy(VX + 2, &c));
+        *out = a + b + c;
+        return SPK_OK;
+    }
+    if (mhd && (v == "em" || v == "Mx" || v == "My" || v == "Mz")) {
+        if (v != "em") return energy(BX + (v[1] - 'x'), out);
+        SPK_TRY(energy(BX, &a)); SPK_TRY(energy(BX + 1, &b)); SPK_TRY(energy(BX + 2, &c));
+        *out = a + b + c;
+        return SPK_OK;
+    }
+    if (th && v == "et") return energy(TH, out);
+    if (inc && v == "vxvy") return corr(VX, VX + 1, out);
+    if (inc && v == "vxvz") return corr(VX, VX + 2, out);
+    if (inc && v == "vyvz") return corr(VX + 1, VX + 2, out);
+    if (mhd && v == "bxby") return corr(BX, BX + 1, out);
+    if (mhd && v == "bxbz") return corr(BX, BX + 2, out);
+    if (mhd && v == "bybz") return corr(BX + 1, BX + 2, out);
+    if (p.boussinesq && v == "thvx") return corr(TH, VX, out);
+    if (p.boussinesq && v == "thvz") return corr(TH, VX + 2, out);
+    if (inc && v == "w2") return diag_spec_sum(s, s->U + (long long)VX * g.ntc, 1, kg, out);
+    if (mhd && v == "j2") return diag_spec_sum(s, s->U + (long long)BX * g.ntc, 1, kg, out);
+    if (mhd && v == "hc") {
+        *out = 0.0;
+        for (int q = 0; q < 3; q++) SPK_TRY(diag_dot(s, R + (VX + q) * fr, R + (BX + q) * fr, out));
+        return SPK_OK;
+    }
+    if ((inc && v == "hv") || (mhd && v == "hm")) {
+        // computeHelicity (spooky_output_functions.cu:88-107)
+        const int f0 = v == "hv" ? VX : BX;
+        SPK_TRY(need_diag());
+        k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)f0 * g.ntc, s->diag, g.ntc, kg, 0);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
+        const double *h = (const double *)s->diag;
+        *out = 0.0;
+        for (int q = 0; q < 3; q++) SPK_TRY(diag_dot(s, h + q * fr, R + (f0 + q) * fr, out));
+        return SPK_OK;
+    }
+    if (mhd && (v == "thetaB" || v == "bz2" || v == "phiB")) {
+        // averagebz / averagebz2 / averagephiB (spooky_output_functions.cu:340-395)
+        const int kind = v == "thetaB" ? 0 : (v == "bz2" ? 1 : 2);
+        const long long rows = (long long)g.nx * g.ny;
+        const unsigned nb = red_grid(rows * g.nz);
+        k_diag_bsum<<<nb, 256, 0, st>>>(R + BX * fr, fr, rows, g.nz, 2 * g.nzh, kind, s->part);
+        SPK_LAUNCH_CHECK();
+        return reduce_final(s, (int)nb, 1.0 / ((double)g.nx * g.ny * g.nz), out);
+    }
+    if (p.boussinesq && inc && v == "pv") {
+        // potentialVorticity (spooky_output_functions.cu:109-140)
+        SPK_TRY(need_diag());
+        k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)VX * g.ntc, s->diag, g.ntc, kg, 1);
+        k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)TH * g.ntc, s->diag + 3 * g.ntc, g.ntc, kg, 2);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 6));
+        const double *h = (const double *)s->diag;
+        *out = 0.0;
+        for (int q = 0; q < 3; q++) SPK_TRY(diag_dot(s, h + q * fr, h + (3 + q) * fr, out));
+        return SPK_OK;
+    }
+    if (th && v == "dissgradT") {
+        if (!(p.anisotropic_diffusion && mhd))  // computeDissipation (spooky_output_functions.cu:142-160)
+            return diag_spec_sum(s, s->U + (long long)TH * g.ntc, 2, kg, out);
+        // - computeAnisoDissipation (spooky_output_functions.cu:228-252, anisotropic_conduction.cu:107-156)
+        SPK_TRY(need_diag());
+        k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)TH * g.ntc, s->diag, g.ntc, kg, 2);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
+        double *h = (double *)s->diag;
+        k_diag_bvec<<<red_grid(fr), 256, 0, st>>>(R + BX * fr, h, h + 3 * fr, fr, p.strat_direction, 1);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, h + 3 * fr, s->diag + 3 * g.ntc, 3));
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag + 3 * g.ntc, s->diag, g.ntc, kg);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 1));
+        double d = 0.0;
+        SPK_TRY(diag_dot(s, h, R + TH * fr, &d));
+        *out = -d;
+        return SPK_OK;
+    }
+    if (p.anisotropic_diffusion && p.boussinesq && v == "fluxbbgradT") {
+        // computeAnisoInjection (spooky_output_functions.cu:254-276, anisotropic_conduction.cu:69-104)
+        SPK_TRY(need_diag());
+        double *h = (double *)s->diag;
+        k_diag_bvec<<<red_grid(fr), 256, 0, st>>>(R + BX * fr, nullptr, h, fr, p.strat_direction, 0);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, h, s->diag, 3));
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag, s->diag + 3 * g.ntc, g.ntc, kg);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag + 3 * g.ntc, s->diag + 3 * g.ntc, 1));
+        *out = 0.0;
+        return diag_dot(s, h + 3 * fr, R + TH * fr, out);
+    }
+    set_error("spk_timevar: '%s' is not a timevar quantity of this configuration", name);
+    return SPK_ERR_ARG;
 }
 
 extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int cap, int *count) {

@@ -166,58 +166,16 @@ struct Run {
         std::fprintf(f, "\n");
         std::fclose(f);
     }
-    double energy(int f) {
-        double e;
-        CALL(spk_energy(sol, f, &e));
-        return e;
-    }
-    double corr(int a, int b) {
-        double e;
-        CALL(spk_correlation(sol, a, b, &e));
-        return e;
-    }
-    double enstrophy(int f0) {
-        double e;
-        CALL(spk_enstrophy(sol, f0, &e));
-        return e;
-    }
     void timevar_row() {
-        const spk_params &p = rp.p;
-        const int VX = vars[0], VY = vars[1], VZ = vars[2], BX = vars[3], BY = vars[4], BZ = vars[5], TH = vars[6];
         CALL(spk_complex2real(sol));
         FILE *f = std::fopen((data_dir + "/timevar.spooky").c_str(), "a");
         if (!f) return;
         for (const auto &v : rp.timevar_vars) {
-            double o = -1.0;  // quantities this build does not compute stay -1, as in the reference
+            // quantities the library does not compute stay -1, as the reference
+            // writes for a name it does not know
+            double o = -1.0;
             if (v == "t") o = tm[0];
-            if (p.incompressible) {
-                if (v == "ev") o = energy(VX) + energy(VY) + energy(VZ);
-                else if (v == "Kx") o = energy(VX);
-                else if (v == "Ky") o = energy(VY);
-                else if (v == "Kz") o = energy(VZ);
-                else if (v == "vxvy") o = corr(VX, VY);
-                else if (v == "vxvz") o = corr(VX, VZ);
-                else if (v == "vyvz") o = corr(VY, VZ);
-                else if (v == "w2") o = enstrophy(VX);
-            }
-            if (p.mhd) {
-                if (v == "em") o = energy(BX) + energy(BY) + energy(BZ);
-                else if (v == "Mx") o = energy(BX);
-                else if (v == "My") o = energy(BY);
-                else if (v == "Mz") o = energy(BZ);
-                else if (v == "bxby") o = corr(BX, BY);
-                else if (v == "bxbz") o = corr(BX, BZ);
-                else if (v == "bybz") o = corr(BY, BZ);
-                else if (v == "j2") o = enstrophy(BX);
-                else if (v == "hc") o = corr(VX, BX) + corr(VY, BY) + corr(VZ, BZ);
-            }
-            if (p.boussinesq || p.heat_equation) {
-                if (v == "et") o = energy(TH);
-            }
-            if (p.boussinesq) {
-                if (v == "thvx") o = corr(TH, VX);
-                else if (v == "thvz") o = corr(TH, VZ);
-            }
+            else if (spk_timevar(sol, v.c_str(), &o) != SPK_OK) o = -1.0;
             std::fprintf(f, "%.8e\t", o);
         }
         std::fprintf(f, "\n");
