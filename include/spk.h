@@ -164,6 +164,31 @@ int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins)
  * Differences: computeAnisoDissipation accumulates into scratch it does not
  * clear (DivergenceMask flag 1); here the divergence is written, not added. */
 int spk_timevar(spk_solver s, const char *name, double *out);
+/* Shell bins of the spectra: nbins and deltak of Wavevector (wavevector.cu:98-110);
+ * bin i is centred on i * deltak, as WriteSpectrumOutputHeader writes
+ * (spooky_output_spectrum.cu:448-480). */
+int spk_spectrum_bins(spk_solver s, int *nbins, double *deltak);
+/* One spectrum of the spectrum file by the reference's name, as
+ * WriteSpectrumOutput computes it (spooky_output_spectrum.cu:22-400): Kx Ky Kz
+ * Mx My Mz Eth vxvy vyvz vxvz bxby bybz bxbz thvx thvy thvz, and the transfer
+ * spectra advec_{x,y,z} emfpower_{x,y,z} lorentz_{x,y,z} th_advec
+ * helicity_{x,y,z} th_aniso_diss th_aniso_inj.  Transfer spectra read the real
+ * buffer, which must hold the current state (spk_complex2real).  Mirrored
+ * quirks: bxbz pairs B_x with B_y (:153-158); th_aniso_diss adds the
+ * anisotropic dissipation to the unmasked transform of B_y B_z, which the
+ * reference's scratch array holds at that point.  SPK_ERR_ARG for a name that
+ * is not a spectrum of this configuration (the reference writes no line). */
+int spk_spectrum_named(spk_solver s, const char *name, double *out, int cap, int *nbins);
+/* One quantity of the user timevar file (WriteUserTimevarOutput,
+ * src/user/user_output_timevar.cu:18-76): kpartheta (computekpartheta,
+ * user_output_functions.cu:31-58; needs anisotropic_diffusion; reads the real
+ * buffer), uservar1 and uservar2 (0, as shipped).  SPK_ERR_ARG otherwise. */
+int spk_user_timevar(spk_solver s, const char *name, double *out);
+/* The real fields a snapshot holds (CheckSnapshot, check_output.cu:70-84) into
+ * the real buffer: Complex2RealFields, or with shearing UnshearOutput
+ * (unshear_output.cu:21-54), which first multiplies a copy of every field by
+ * exp(-i ky Lx S fmod(t, 2 Ly / (S Lx)) / 2).  Synchronizes to read t. */
+int spk_snapshot_real(spk_solver s);
 
 #ifdef __cplusplus
 }

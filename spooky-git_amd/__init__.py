@@ -105,6 +105,10 @@ SIGNATURES = {
     "spk_correlation": [_VP, _I, _I, _VP],
     "spk_spectrum": [_VP, _I, _I, _VP, _I, _VP],
     "spk_timevar": [_VP, ctypes.c_char_p, _VP],
+    "spk_spectrum_bins": [_VP, _VP, _VP],
+    "spk_spectrum_named": [_VP, ctypes.c_char_p, _VP, _I, _VP],
+    "spk_user_timevar": [_VP, ctypes.c_char_p, _VP],
+    "spk_snapshot_real": [_VP],
 }
 
 
@@ -371,6 +375,31 @@ class Solver:
         nb = ctypes.c_int()
         check(lib().spk_spectrum(self.h, fa, fb, out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(nb)))
         return out[: nb.value].copy()
+
+    def spectrum_bins(self):
+        """(nbins, deltak) of the shell spectra."""
+        nb, dk = ctypes.c_int(), ctypes.c_double()
+        check(lib().spk_spectrum_bins(self.h, ctypes.byref(nb), ctypes.byref(dk)))
+        return nb.value, dk.value
+
+    def spectrum_named(self, name):
+        """One spectrum of the spectrum file by the reference's name (Kx, bxbz,
+        advec_x, emfpower_y, th_aniso_diss, ...); transfer spectra read the real
+        buffer, which must hold the current state (complex2real)."""
+        nb = self.spectrum_bins()[0]
+        out = np.zeros(max(nb, 1))
+        check(lib().spk_spectrum_named(self.h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), nb, None))
+        return out[:nb].copy()
+
+    def user_timevar(self, name):
+        """One quantity of the user timevar file (kpartheta, uservar1, uservar2)."""
+        o = ctypes.c_double()
+        check(lib().spk_user_timevar(self.h, name.encode(), ctypes.byref(o)))
+        return o.value
+
+    def snapshot_real(self):
+        """The real fields a snapshot holds into the real buffer (unsheared in a shearing box)."""
+        check(lib().spk_snapshot_real(self.h))
 
 
 def padded_real(r):

@@ -426,9 +426,10 @@ __global__ __launch_bounds__(256) void k_diag_bvec(const double *__restrict__ B,
     }
 }
 
-// DivergenceMask (cuda_kernels_generic.cu:315-334): i mask (k . X)
+// DivergenceMask (cuda_kernels_generic.cu:315-334): i mask (k . X), written
+// (acc 0, flag 0 there) or added to Z (acc 1, flag 1 there)
 __global__ __launch_bounds__(256) void k_diag_div(const cpx<double> *__restrict__ X, cpx<double> *__restrict__ Z,
-                                                  long long ntc, KGeom kg) {
+                                                  long long ntc, KGeom kg, int acc) {
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
          idx += (long long)gridDim.x * blockDim.x) {
         const long long rowi = idx / kg.nzh;
@@ -437,7 +438,8 @@ __global__ __launch_bounds__(256) void k_diag_div(const cpx<double> *__restrict_
         double kx, ky, kz, m;
         kvec_at(kg, i, j, k, kx, ky, kz, m);
         const cpx<double> d = kdot3(kx, X[idx], ky, X[ntc + idx], kz, X[2 * ntc + idx]);
-        Z[idx] = cpx<double>{-m * d.y, m * d.x};
+        const cpx<double> z0 = acc ? Z[idx] : cpx<double>{0.0, 0.0};
+        Z[idx] = cpx<double>{z0.x - m * d.y, z0.y + m * d.x};
     }
 }
 
@@ -464,6 +466,99 @@ __global__ __launch_bounds__(256) void k_diag_bsum(const double *__restrict__ B,
     }
     const double s = block_sum(acc, sm);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---- spectral transfer terms of the spectrum output (spooky_output_spectrum.cu:22-400) ----
+
+// real-space products over padded rows (padding included, as there); inputs and
+// outputs at a stride of n doubles per component:
+// kind 0 ShearMatrix (cuda_kernels.cu:59-77): A_i A_j for ij = xx xy xz yy yz zz
+// kind 1 MagneticEmf (cuda_kernels.cu:240-252): A x B with A = u, B = B
+// kind 2 EnergyFluxVector (cuda_kernels.cu:277-290): A_i B with B a scalar
+// kind 3 A_1 A_2, one output (B_y B_z)
+// kind 4 ComputebGradTheta (user_defined_cuda_kernels.cu:50-60): B = real grad
+//   theta, A = real B: (A . B) / (1e-16 + |A|), one output
+__global__ __launch_bounds__(256) void k_diag_prod(const double *A, const double *B, double *Z, long long n, int kind) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double a0 = A[i], a1 = A[n + i], a2 = kind == 2 || kind == 3 ? 0.0 : A[2 * n + i];
+        if (kind == 0) {
+            Z[i] = a0 * a0;
+            Z[n + i] = a0 * a1;
+            Z[2 * n + i] = a0 * a2;
+            Z[3 * n + i] = a1 * a1;
+            Z[4 * n + i] = a1 * a2;
+            Z[5 * n + i] = a2 * a2;
+        } else if (kind == 1) {
+            const double b0 = B[i], b1 = B[n + i], b2 = B[2 * n + i];
+            Z[i] = -b1 * a2 + b2 * a1;
+            Z[n + i] = b0 * a2 - b2 * a0;
+            Z[2 * n + i] = -b0 * a1 + b1 * a0;
+        } else if (kind == 2) {
+            const double th = B[i];
+            Z[i] = a0 * th;
+            Z[n + i] = a1 * th;
+            Z[2 * n + i] = A[2 * n + i] * th;
+        } else if (kind == 3) {
+            Z[i] = a1 * A[2 * n + i];
+        } else {
+            const double nrm = 1e-16 + sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            Z[i] = (a0 * B[i] + a1 * B[n + i] + a2 * B[2 * n + i]) / nrm;
+        }
+    }
+}
+
+// spectral operators on transformed products, in place (Z may equal X: every
+// element reads its inputs before it writes):
+// kind 0 NonLinAdvection (cuda_kernels.cu:101-123): Z_i = -i mask k_j S_ij from
+//   the 6 components of kind 0 above
+// kind 1 MagneticShear (cuda_kernels.cu:257-272): Z = i mask k x emf
+// kind 2 NonLinBoussinesqAdv (cuda_kernels.cu:293-303): Z = -i mask k . E
+__global__ __launch_bounds__(256) void k_diag_sop(const cpx<double> *X, cpx<double> *Z, long long ntc, KGeom kg,
+                                                  int kind) {
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+        double kx, ky, kz, m;
+        kvec_at(kg, i, j, k, kx, ky, kz, m);
+        const cpx<double> x0 = X[idx], x1 = X[ntc + idx], x2 = X[2 * ntc + idx];
+        if (kind == 0) {
+            const cpx<double> x3 = X[3 * ntc + idx], x4 = X[4 * ntc + idx], x5 = X[5 * ntc + idx];
+            const cpx<double> d0 = kdot3(kx, x0, ky, x1, kz, x2);
+            const cpx<double> d1 = kdot3(kx, x1, ky, x3, kz, x4);
+            const cpx<double> d2 = kdot3(kx, x2, ky, x4, kz, x5);
+            Z[idx] = cpx<double>{m * d0.y, -m * d0.x};
+            Z[ntc + idx] = cpx<double>{m * d1.y, -m * d1.x};
+            Z[2 * ntc + idx] = cpx<double>{m * d2.y, -m * d2.x};
+        } else if (kind == 1) {
+            const cpx<double> c0 = {ky * x2.x - kz * x1.x, ky * x2.y - kz * x1.y};
+            const cpx<double> c1 = {kz * x0.x - kx * x2.x, kz * x0.y - kx * x2.y};
+            const cpx<double> c2 = {kx * x1.x - ky * x0.x, kx * x1.y - ky * x0.y};
+            Z[idx] = cpx<double>{-m * c0.y, m * c0.x};
+            Z[ntc + idx] = cpx<double>{-m * c1.y, m * c1.x};
+            Z[2 * ntc + idx] = cpx<double>{-m * c2.y, m * c2.x};
+        } else {
+            const cpx<double> d = kdot3(kx, x0, ky, x1, kz, x2);
+            Z[idx] = cpx<double>{m * d.y, -m * d.x};
+        }
+    }
+}
+
+// UnshearComplexVec (cuda_kernels_generic.cu:425-439) on a copy of nf fields:
+// Z = U exp(-i ky c / 2), ky unsheared
+__global__ __launch_bounds__(256) void k_unshear(const cpx<double> *__restrict__ U, cpx<double> *__restrict__ Z,
+                                                 long long ntc, long long n, KGeom kg, double c) {
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long e = idx % ntc;
+        const int j = (int)((e / kg.nzh) % kg.ny);
+        const int jj = j < kg.ny / 2 ? j : j - kg.ny;
+        const double ph = -0.5 * (kg.kfy * (double)jj) * c;
+        const double cs = cos(ph), sn = sin(ph);
+        const cpx<double> u = U[idx];
+        Z[idx] = cpx<double>{u.x * cs - u.y * sn, u.x * sn + u.y * cs};
+    }
 }
 
 }  // namespace spk
@@ -1408,23 +1503,38 @@ extern "C" int spk_correlation(spk_solver s, int fa, int fb, double *out) {
     return reduce_final(s, (int)nb, 1.0 / ((double)s->g.nx * s->g.ny * s->g.nz), out);
 }
 
-extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins) {
-    SPK_CHECK_SOLVER(s);
-    SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out && cap >= 0, SPK_ERR_ARG,
-                "spk_spectrum: bad arguments");
+namespace {
+
+// computeSpectrum1d (spooky_output_spectrum.cu:424-444) of two spectral fields
+int spectrum_of(spk_solver s, const cpx<double> *a, const cpx<double> *b, double *out, int cap) {
     const int nb = s->nbins;
-    if (nbins) *nbins = nb;
-    SPK_REQUIRE(nb <= 16384, SPK_ERR_ARG, "spk_spectrum: %d bins exceed the LDS histogram", nb);
+    SPK_REQUIRE(nb <= 16384, SPK_ERR_ARG, "spectrum: %d bins exceed the LDS histogram", nb);
     k_zero_d<<<1, 256, 0, s->ctx->stream>>>(s->spec, nb + 1);
     const unsigned g = red_grid(s->g.ntc);
     k_spectrum<<<g, 256, sizeof(double) * (size_t)(nb + 1), s->ctx->stream>>>(
-        s->U + (long long)fa * s->g.ntc, s->U + (long long)fb * s->g.ntc, s->g.ntc, kgeom(s, s->tremap * s->p.shear),
-        (double)s->g.nx * s->g.ny * s->g.nz, s->deltak, nb, s->spec);
+        a, b, s->g.ntc, kgeom(s, s->tremap * s->p.shear), (double)s->g.nx * s->g.ny * s->g.nz, s->deltak, nb, s->spec);
     SPK_LAUNCH_CHECK();
     std::vector<double> h((size_t)nb + 1);
     SPK_HIP(hipMemcpyAsync(h.data(), s->spec, sizeof(double) * (size_t)(nb + 1), hipMemcpyDeviceToHost, s->ctx->stream));
     SPK_HIP(hipStreamSynchronize(s->ctx->stream));
     for (int i = 0; i < nb && i < cap; i++) out[i] = h[i];
+    return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out && cap >= 0, SPK_ERR_ARG,
+                "spk_spectrum: bad arguments");
+    if (nbins) *nbins = s->nbins;
+    return spectrum_of(s, s->U + (long long)fa * s->g.ntc, s->U + (long long)fb * s->g.ntc, out, cap);
+}
+
+extern "C" int spk_spectrum_bins(spk_solver s, int *nbins, double *deltak) {
+    SPK_CHECK_SOLVER(s);
+    if (nbins) *nbins = s->nbins;
+    if (deltak) *deltak = s->deltak;
     return SPK_OK;
 }
 
@@ -1462,6 +1572,19 @@ int diag_dot(spk_solver s, const double *a, const double *b, double *acc) {
     return SPK_OK;
 }
 
+// 6 FP64 fields of scratch, allocated at the first diagnostic that needs them
+int need_diag(spk_solver s) {
+    if (!s->diag) {
+        hipError_t e = hipMalloc((void **)&s->diag, (size_t)s->g.ntc * 16 * 6);
+        if (e != hipSuccess) {
+            s->diag = nullptr;
+            set_error("hipMalloc(diagnostic scratch): %s", hipGetErrorString(e));
+            return SPK_ERR_ALLOC;
+        }
+    }
+    return SPK_OK;
+}
+
 int diag_spec_sum(spk_solver s, const cpx<double> *u, int kind, const KGeom &kg, double *out) {
     const unsigned nb = red_grid(s->g.ntc);
     k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(u, s->g.ntc, kg, kind, s->part);
@@ -1487,17 +1610,6 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
     auto corr = [&](int a, int b, double *o) {
         *o = 0.0;
         return diag_dot(s, R + a * fr, R + b * fr, o);
-    };
-    auto need_diag = [&]() -> int {
-        if (!s->diag) {
-            hipError_t e = hipMalloc((void **)&s->diag, (size_t)g.ntc * 16 * 6);
-            if (e != hipSuccess) {
-                s->diag = nullptr;
-                set_error("hipMalloc(diagnostic scratch): %s", hipGetErrorString(e));
-                return SPK_ERR_ALLOC;
-            }
-        }
-        return SPK_OK;
     };
     const unsigned gb = red_grid(g.ntc);
     hipStream_t st = s->ctx->stream;
@@ -1534,7 +1646,7 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
     if ((inc && v == "hv") || (mhd && v == "hm")) {
         // computeHelicity (spooky_output_functions.cu:88-107)
         const int f0 = v == "hv" ? VX : BX;
-        SPK_TRY(need_diag());
+        SPK_TRY(need_diag(s));
         k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)f0 * g.ntc, s->diag, g.ntc, kg, 0);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
@@ -1554,7 +1666,7 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
     }
     if (p.boussinesq && inc && v == "pv") {
         // potentialVorticity (spooky_output_functions.cu:109-140)
-        SPK_TRY(need_diag());
+        SPK_TRY(need_diag(s));
         k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)VX * g.ntc, s->diag, g.ntc, kg, 1);
         k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)TH * g.ntc, s->diag + 3 * g.ntc, g.ntc, kg, 2);
         SPK_LAUNCH_CHECK();
@@ -1568,7 +1680,7 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
         if (!(p.anisotropic_diffusion && mhd))  // computeDissipation (spooky_output_functions.cu:142-160)
             return diag_spec_sum(s, s->U + (long long)TH * g.ntc, 2, kg, out);
         // - computeAnisoDissipation (spooky_output_functions.cu:228-252, anisotropic_conduction.cu:107-156)
-        SPK_TRY(need_diag());
+        SPK_TRY(need_diag(s));
         k_diag_vec<<<gb, 256, 0, st>>>(s->U + (long long)TH * g.ntc, s->diag, g.ntc, kg, 2);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
@@ -1576,7 +1688,7 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
         k_diag_bvec<<<red_grid(fr), 256, 0, st>>>(R + BX * fr, h, h + 3 * fr, fr, p.strat_direction, 1);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_r2c(s, h + 3 * fr, s->diag + 3 * g.ntc, 3));
-        k_diag_div<<<gb, 256, 0, st>>>(s->diag + 3 * g.ntc, s->diag, g.ntc, kg);
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag + 3 * g.ntc, s->diag, g.ntc, kg, 0);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_c2r(s, s->diag, s->diag, 1));
         double d = 0.0;
@@ -1586,12 +1698,12 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
     }
     if (p.anisotropic_diffusion && p.boussinesq && v == "fluxbbgradT") {
         // computeAnisoInjection (spooky_output_functions.cu:254-276, anisotropic_conduction.cu:69-104)
-        SPK_TRY(need_diag());
+        SPK_TRY(need_diag(s));
         double *h = (double *)s->diag;
         k_diag_bvec<<<red_grid(fr), 256, 0, st>>>(R + BX * fr, nullptr, h, fr, p.strat_direction, 0);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_r2c(s, h, s->diag, 3));
-        k_diag_div<<<gb, 256, 0, st>>>(s->diag, s->diag + 3 * g.ntc, g.ntc, kg);
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag, s->diag + 3 * g.ntc, g.ntc, kg, 0);
         SPK_LAUNCH_CHECK();
         SPK_TRY(diag_c2r(s, s->diag + 3 * g.ntc, s->diag + 3 * g.ntc, 1));
         *out = 0.0;
@@ -1599,6 +1711,176 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
     }
     set_error("spk_timevar: '%s' is not a timevar quantity of this configuration", name);
     return SPK_ERR_ARG;
+}
+
+// WriteSpectrumOutput (spooky_output_spectrum.cu:22-400), one spectrum by name
+extern "C" int spk_spectrum_named(spk_solver s, const char *name, double *out, int cap, int *nbins) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(name && out && cap >= 0, SPK_ERR_ARG, "spk_spectrum_named: bad arguments");
+    if (nbins) *nbins = s->nbins;
+    const std::string v(name);
+    const spk_params &p = s->p;
+    const Grid3 &g = s->g;
+    const long long fr = 2 * g.ntc;  // doubles per padded real field
+    const KGeom kg = kgeom(s, s->tremap * p.shear);
+    const double *R = s->realbuf;
+    const int VX = s->VX, BX = s->BX, TH = s->TH;
+    const bool inc = p.incompressible, mhd = p.mhd, bq = p.boussinesq;
+    auto F = [&](int f) -> const cpx<double> * { return s->U + (long long)f * g.ntc; };
+    // shell spectra of two state fields; bxbz pairs B_x with B_y as the reference
+    // does (spooky_output_spectrum.cu:153-158)
+    struct Pair {
+        const char *n;
+        bool on;
+        int a, b;
+    };
+    const Pair pairs[] = {
+        {"Kx", inc, VX, VX},         {"Ky", inc, VX + 1, VX + 1},     {"Kz", inc, VX + 2, VX + 2},
+        {"Mx", mhd, BX, BX},         {"My", mhd, BX + 1, BX + 1},     {"Mz", mhd, BX + 2, BX + 2},
+        {"Eth", bq || p.heat_equation, TH, TH},
+        {"vxvy", inc, VX, VX + 1},   {"vyvz", inc, VX + 1, VX + 2},   {"vxvz", inc, VX, VX + 2},
+        {"bxby", mhd, BX, BX + 1},   {"bybz", mhd, BX + 1, BX + 2},   {"bxbz", mhd, BX, BX + 1},
+        {"thvx", bq, TH, VX},        {"thvy", bq, TH, VX + 1},        {"thvz", bq, TH, VX + 2},
+    };
+    for (const Pair &q : pairs)
+        if (q.on && v == q.n) return spectrum_of(s, F(q.a), F(q.b), out, cap);
+    // transfer spectra: a state field against a transformed nonlinear term
+    auto comp = [&](const char *base) -> int {
+        const size_t nb = strlen(base);
+        if (v.size() == nb + 1 && v.compare(0, nb, base) == 0 && v[nb] >= 'x' && v[nb] <= 'z') return v[nb] - 'x';
+        return -1;
+    };
+    hipStream_t st = s->ctx->stream;
+    const unsigned gb = red_grid(g.ntc), gr = red_grid(fr);
+    int c;
+    if ((inc && (c = comp("advec_")) >= 0) || (inc && mhd && (c = comp("lorentz_")) >= 0)) {
+        // NonLinearAdvection (hydro.cu:78-105) of real u (advec) or real B (lorentz),
+        // against u
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_prod<<<gr, 256, 0, st>>>(R + (v[0] == 'a' ? VX : BX) * fr, nullptr, Dr, fr, 0);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr, s->diag, 6));
+        k_diag_sop<<<gb, 256, 0, st>>>(s->diag, s->diag, g.ntc, kg, 0);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(VX + c), s->diag + c * g.ntc, out, cap);
+    }
+    if (inc && mhd && (c = comp("emfpower_")) >= 0) {
+        // CurlEMF (mhd.cu:20-62) against B
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_prod<<<gr, 256, 0, st>>>(R + VX * fr, R + BX * fr, Dr, fr, 1);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr, s->diag, 3));
+        k_diag_sop<<<gb, 256, 0, st>>>(s->diag, s->diag, g.ntc, kg, 1);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(BX + c), s->diag + c * g.ntc, out, cap);
+    }
+    if (bq && inc && v == "th_advec") {
+        // AdvectTemperature (boussinesq.cu:10-42) against theta
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_prod<<<gr, 256, 0, st>>>(R + VX * fr, R + TH * fr, Dr, fr, 2);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr, s->diag, 3));
+        k_diag_sop<<<gb, 256, 0, st>>>(s->diag, s->diag, g.ntc, kg, 2);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(TH), s->diag, out, cap);
+    }
+    if (mhd && (c = comp("helicity_")) >= 0) {
+        // MagneticHelicity (mhd.cu:65-76) against B
+        SPK_TRY(need_diag(s));
+        k_diag_vec<<<gb, 256, 0, st>>>(F(BX), s->diag, g.ntc, kg, 0);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(BX + c), s->diag + c * g.ntc, out, cap);
+    }
+    if (p.anisotropic_diffusion && v == "th_aniso_diss") {
+        // AnisotropicDissipation (anisotropic_conduction.cu:107-156). Its last step
+        // (DivergenceMask, flag 1) adds to a scratch array that, at this point of
+        // WriteSpectrumOutput, still holds the unmasked transform of B_y B_z left
+        // there by the lorentz term (hydro.cu:95-101 on real B); the sum is mirrored.
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_vec<<<gb, 256, 0, st>>>(F(TH), s->diag, g.ntc, kg, 2);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
+        k_diag_bvec<<<gr, 256, 0, st>>>(R + BX * fr, Dr, Dr + 3 * fr, fr, p.strat_direction, 1);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr + 3 * fr, s->diag + 3 * g.ntc, 3));
+        k_diag_prod<<<gr, 256, 0, st>>>(R + BX * fr, nullptr, Dr, fr, 3);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr, s->diag, 1));
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag + 3 * g.ntc, s->diag, g.ntc, kg, 1);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(TH), s->diag, out, cap);
+    }
+    if (p.anisotropic_diffusion && v == "th_aniso_inj") {
+        // AnisotropicInjection (anisotropic_conduction.cu:69-104)
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_bvec<<<gr, 256, 0, st>>>(R + BX * fr, nullptr, Dr, fr, p.strat_direction, 0);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_r2c(s, Dr, s->diag, 3));
+        k_diag_div<<<gb, 256, 0, st>>>(s->diag, s->diag + 3 * g.ntc, g.ntc, kg, 0);
+        SPK_LAUNCH_CHECK();
+        return spectrum_of(s, F(TH), s->diag + 3 * g.ntc, out, cap);
+    }
+    set_error("spk_spectrum_named: '%s' is not a spectrum of this configuration", name);
+    return SPK_ERR_ARG;
+}
+
+// WriteUserTimevarOutput (user_output_timevar.cu:18-76): the user quantities the
+// reference ships
+extern "C" int spk_user_timevar(spk_solver s, const char *name, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(name && out, SPK_ERR_ARG, "spk_user_timevar: null argument");
+    const std::string v(name);
+    const spk_params &p = s->p;
+    const Grid3 &g = s->g;
+    if (p.anisotropic_diffusion && v == "kpartheta") {
+        // computekpartheta (user_output_functions.cu:31-58): <(b . grad theta)^2> / (2 E_theta)
+        const long long fr = 2 * g.ntc;
+        const KGeom kg = kgeom(s, s->tremap * p.shear);
+        hipStream_t st = s->ctx->stream;
+        SPK_TRY(need_diag(s));
+        double *Dr = (double *)s->diag;
+        k_diag_vec<<<red_grid(g.ntc), 256, 0, st>>>(s->U + (long long)s->TH * g.ntc, s->diag, g.ntc, kg, 2);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(diag_c2r(s, s->diag, s->diag, 3));
+        k_diag_prod<<<red_grid(fr), 256, 0, st>>>(s->realbuf + s->BX * fr, Dr, Dr + 3 * fr, fr, 4);
+        SPK_LAUNCH_CHECK();
+        double c2 = 0.0, e = 0.0;
+        SPK_TRY(diag_dot(s, Dr + 3 * fr, Dr + 3 * fr, &c2));
+        SPK_TRY(diag_spec_sum(s, s->U + (long long)s->TH * g.ntc, 0, kg, &e));
+        *out = c2 / (2.0 * e);
+        return SPK_OK;
+    }
+    // customFunction returns 0 (user_output_functions.cu:60-84); uservar2 is 0
+    if (v == "uservar1" || v == "uservar2") {
+        *out = 0.0;
+        return SPK_OK;
+    }
+    set_error("spk_user_timevar: '%s' is not a user quantity of this configuration", name);
+    return SPK_ERR_ARG;
+}
+
+// the real fields a snapshot holds (check_output.cu:70-84): the inverse transform
+// of the state, unsheared first in a shearing box (UnshearOutput,
+// unshear_output.cu:21-54: phase exp(-i ky c / 2), c = Lx S fmod(t, 2 Ly / (S Lx)))
+extern "C" int spk_snapshot_real(spk_solver s) {
+    SPK_CHECK_SOLVER(s);
+    const spk_params &p = s->p;
+    if (!p.shearing) return spk_complex2real(s);
+    const Grid3 &g = s->g;
+    TimeState h;
+    SPK_TRY(read_time(s, &h));
+    const double tvel = fmod(h.t, 2.0 * p.ly / (p.shear * p.lx));
+    const long long n = (long long)s->nf * g.ntc;
+    k_unshear<<<red_grid(n), 256, 0, s->ctx->stream>>>(s->U, reinterpret_cast<cpx<double> *>(s->realbuf), g.ntc, n,
+                                                       s->kg0, tvel * p.lx * p.shear);
+    SPK_LAUNCH_CHECK();
+    s->products_valid = false;
+    return diag_c2r(s, s->realbuf, s->realbuf, s->nf);
 }
 
 extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int cap, int *count) {

@@ -79,7 +79,7 @@ struct Run {
     void write_snapshot() {
         const spk_params &p = rp.p;
         const size_t n = (size_t)p.nx * p.ny * p.nz;
-        CALL(spk_complex2real(sol));
+        CALL(spk_snapshot_real(sol));  // unsheared in a shearing box
         host_real.resize(2 * ntc * nf);
         CALL(spk_memcpy_d2h(ctx, host_real.data(), d_real, sizeof(double) * host_real.size()));
         std::vector<std::vector<double>> f(nf, std::vector<double>(n));
@@ -167,7 +167,6 @@ struct Run {
         std::fclose(f);
     }
     void timevar_row() {
-        CALL(spk_complex2real(sol));
         FILE *f = std::fopen((data_dir + "/timevar.spooky").c_str(), "a");
         if (!f) return;
         for (const auto &v : rp.timevar_vars) {
@@ -182,15 +181,76 @@ struct Run {
         std::fclose(f);
     }
 
-    // CheckTimeSeries (check_output.cu:15-51)
+    // WriteSpectrumOutputHeader / WriteSpectrumOutput (spooky_output_spectrum.cu:22-480)
+    void spectrum_header() {
+        int nb = 0;
+        double dk = 0.0;
+        CALL(spk_spectrum_bins(sol, &nb, &dk));
+        FILE *f = std::fopen((data_dir + "/spectrum.spooky").c_str(), "a");
+        if (!f) return;
+        std::fprintf(f, "## This file contains the 1d (shell-integrated) energy spectral densities of the following "
+                        "quantities: \n## \t");
+        for (const auto &v : rp.spectra_vars) std::fprintf(f, "%s\t", v.c_str());
+        std::fprintf(f, "\nwavevector \t");
+        for (int i = 0; i < nb; i++) std::fprintf(f, "%.8e\t", i * dk);
+        std::fprintf(f, "\n");
+        std::fclose(f);
+    }
+    void spectrum_rows() {
+        if (rp.spectra_vars.empty()) return;
+        int nb = 0;
+        CALL(spk_spectrum_bins(sol, &nb, nullptr));
+        std::vector<double> sp((size_t)nb);
+        FILE *f = std::fopen((data_dir + "/spectrum.spooky").c_str(), "a");
+        if (!f) return;
+        for (const auto &v : rp.spectra_vars) {
+            // a name the library does not compute gets no line, as in the reference
+            if (spk_spectrum_named(sol, v.c_str(), sp.data(), nb, nullptr) != SPK_OK) continue;
+            std::fprintf(f, "t\t%.8e\t%s\t", tm[0], v.c_str());
+            for (double x : sp) std::fprintf(f, "%.8e\t", x);
+            std::fprintf(f, "\n");
+        }
+        std::fclose(f);
+    }
+    // WriteUserTimevarOutputHeader / WriteUserTimevarOutput (user_output_timevar.cu:18-112)
+    void user_header() {
+        FILE *f = std::fopen((data_dir + "/user-timevar.spooky").c_str(), "a");
+        if (!f) return;
+        std::fprintf(f, "## This file contains the time evolution of the following quantities: \n## \t");
+        for (const auto &v : rp.user_timevar_vars) std::fprintf(f, "%s\t", v.c_str());
+        std::fprintf(f, "\n");
+        std::fclose(f);
+    }
+    void user_row() {
+        FILE *f = std::fopen((data_dir + "/user-timevar.spooky").c_str(), "a");
+        if (!f) return;
+        for (const auto &v : rp.user_timevar_vars) {
+            double o = -1.0;
+            if (spk_user_timevar(sol, v.c_str(), &o) != SPK_OK) o = -1.0;
+            std::fprintf(f, "%.8e\t", o);
+        }
+        std::fprintf(f, "\n");
+        std::fclose(f);
+    }
+
+    // CheckTimeSeries (check_output.cu:23-57); the rows read the real fields of
+    // the state, made once here
     void check_time_series() {
         const double t = tm[0];
         if ((t - t_lastvar) >= rp.toutput_time || t == 0.0) {
             CALL(spk_clean_divergence(sol));
             std::printf("Saving output at t = %.6e \t and step n. %d \n", t, (int)tm[7]);
-            if (t == 0.0) timevar_header();
+            const bool user = !rp.user_timevar_vars.empty();
+            if (t == 0.0) {
+                timevar_header();
+                spectrum_header();
+                if (user) user_header();
+            }
             if (t != 0.0) t_lastvar += rp.toutput_time;
-            if (!rp.timevar_vars.empty()) timevar_row();
+            CALL(spk_complex2real(sol));
+            timevar_row();
+            spectrum_rows();
+            if (user) user_row();
         }
     }
     // CheckSnapshot (check_output.cu:53-94); due comes from the device snapshot

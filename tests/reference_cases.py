@@ -64,3 +64,54 @@ def advected_mhd_vortex(t, n=(512, 512, 2)):
     g = np.exp((1.0 - sigma * (Xt ** 2 + Yt ** 2)) / 2.0) / TWO_PI
     zero = 0 * X
     return {"vx": v0 - g * Yt, "vy": u0 + g * Xt, "vz": zero, "bx": -g * Yt, "by": g * Xt, "bz": zero}
+
+
+def sheared(w, t, shear, L):
+    """w(x, y + shear x t) on the periodic grid, by a y-spectral phase
+    exp(i ky x shear t) (the analytic shearing of check-shearing-wave.py:60-63)."""
+    n = w.shape
+    X = grid(n, L)[0]
+    ky = TWO_PI * np.fft.fftfreq(n[1], d=L[1] / n[1])[None, :, None]
+    return np.real(np.fft.ifft(np.fft.fft(w, axis=1) * np.exp(1j * ky * X * shear * t), axis=1))
+
+
+def shearing_wave(t, n=(64, 64, 64)):
+    """check-shearing-wave.py:14-45 and :104-113: vz = cos(ky0 y) sheared by the
+    background flow, decaying viscously as exp(-nu (ky0^2 t + S^2 ky0^2 t^3 / 3))
+    (kx0 = kz0 = 0), nu = 1/2000, S = 1.5, L = (1, 1.5, 1)."""
+    L = (1.0, 1.5, 1.0)
+    X, Y, Z = grid(n, L)
+    shear, nu, ky0 = 1.5, 1.0 / 2000.0, TWO_PI / L[1]
+    decay = np.exp(-nu * (ky0 ** 2 * t + shear ** 2 * ky0 ** 2 * t ** 3 / 3.0))
+    zero = 0 * X
+    return {"vx": zero, "vy": zero, "vz": sheared(np.cos(ky0 * Y), t, shear, L) * decay}
+
+
+def mri_eigenmode(t, n=(64, 64, 64)):
+    """check-mri-eigenmode.py:14-74 and :118-135: the viscous-resistive MRI
+    channel mode (kx = 2 pi, kz = 4 pi, B0z = 0.1, Omega = 1, S = 1.5,
+    nu = eta = 1/3000) growing as exp(sigma t); bz includes the mean B0z."""
+    L = (1.0, 1.5, 1.0)
+    X, Y, Z = grid(n, L)
+    S, Om, B0z, re = 1.5, 1.0, 0.1, 3000.0
+    kZ, kX = 2.0 * TWO_PI, TWO_PI
+    gamma2 = kZ ** 2 / (kZ ** 2 + kX ** 2)
+    kappa2 = 2 * Om * (2 * Om - S)
+    wA2 = B0z ** 2 * kZ ** 2
+    w2 = wA2 + 0.5 * kappa2 * gamma2 * (1.0 - np.sqrt(1.0 + 16.0 * wA2 * Om ** 2 / (kappa2 ** 2 * gamma2)))
+    s_nu = np.sqrt(-w2)
+    sigma = s_nu - (kX ** 2 + kZ ** 2) / re
+    s_eta = sigma + (kX ** 2 + kZ ** 2) / re
+    a = 1e-4
+    ph = kX * X + kZ * Z
+    vx = a * np.cos(ph)
+    vy = vx * (S * wA2 / s_eta ** 2 - (2 * Om - S)) / (s_nu + wA2 / s_eta)
+    vz = -(kX / kZ) * a * np.cos(ph)
+    bx = -a * np.sin(ph) * B0z * kZ / s_eta
+    by = bx * (-S / s_eta + (S * wA2 / s_eta ** 2 - (2 * Om - S)) / (s_nu + wA2 / s_eta))
+    bzp = B0z * (kX / kZ) * a * np.sin(ph) * kZ / s_eta
+    g = np.exp(sigma * t)
+    out = {k: sheared(v, t, S, L) * g for k, v in (("vx", vx), ("vy", vy), ("vz", vz), ("bx", bx), ("by", by),
+                                                      ("bz", bzp))}
+    out["bz"] = out["bz"] + B0z
+    return out

@@ -167,3 +167,99 @@ def test_advected_mhd_vortex(tmp_path):
     for hdr, f in snaps:
         err = rc.l1(f, rc.advected_mhd_vortex(hdr["t_save"]), names)
         assert err < 1e-4, (hdr, err)
+
+
+def test_shearing_wave(tmp_path):
+    """tests/shearing-wave (shearing box, S = 1.5): the snapshots hold the
+    unsheared fields (UnshearOutput, unshear_output.cu:21-54) and match the
+    sheared, viscously decaying wave with L1 < 1e-7 (check-shearing-wave.py:38,
+    :104-118) at every snapshot up to t_final = 10."""
+    out = str(tmp_path)
+    names = ["vx", "vy", "vz"]
+    write_ic(out, rc.shearing_wave(0.0))
+    run(os.path.join(CFG, "shearing-wave"), out, "-r", "0")
+    snaps = read_all(out, names, (64, 64, 64))
+    assert len(snaps) == 11 and snaps[-1][0]["t_save"] == pytest.approx(10.0, abs=1e-12)
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.shearing_wave(hdr["t_save"]), names)
+        assert err < 1e-7, (hdr, err)
+
+
+def shell_box(a, b, n, L, nb):
+    """Spectrum1d (cuda_kernels_generic.cu:441-471) in numpy on an n, L box:
+    Re(a b*) / N^2 per half-space mode (k_z > 0 twice), shells of deltak = 2 pi / min L."""
+    k = [rc.TWO_PI / Li * np.fft.fftfreq(ni, 1.0 / ni) for ni, Li in zip(n[:2], L[:2])]
+    KX, KY, KZ = np.meshgrid(k[0], k[1], rc.TWO_PI / L[2] * np.arange(n[2] // 2 + 1), indexing="ij")
+    pw = np.real(a * np.conj(b)) / float(np.prod(n)) ** 2
+    pw[:, :, 1:] *= 2.0
+    bins = (np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2) / (rc.TWO_PI / min(L)) + 0.5).astype(int)
+    keep = bins < nb
+    return np.bincount(bins[keep], weights=pw[keep], minlength=nb)[:nb]
+
+
+
+def test_mri_eigenmode_and_spectrum_file(tmp_path):
+    """tests/mri-eigenmode (shearing box with B0z = 0.1): L1 < 1e-7
+    (check-mri-eigenmode.py:76, :118-135) at every snapshot.  The same run,
+    with spectra and user quantities added to its output section, writes
+    data/spectrum.spooky and data/user-timevar.spooky as the reference does
+    (WriteSpectrumOutputHeader / writeSpectrumHelper,
+    spooky_output_spectrum.cu:403-480; user_output_timevar.cu:18-112), read
+    here the way tests/mti-eigenmode/plot_spectrum.py reads it."""
+    cfg = tmp_path / "cfg"
+    cfg.mkdir()
+    text = open(os.path.join(CFG, "mri-eigenmode", "spooky.cfg")).read()
+    anchor = '"bxby","bxbz","bybz");'
+    assert anchor in text
+    spectra = ["Kx", "Mz", "bxbz", "lorentz_y", "Eth", "emfpower_y"]
+    text = text.replace(anchor, anchor + '\n\tspectra_vars = (%s);\n\tuser_timevar_vars = ("kpartheta", "uservar1");'
+                        % ", ".join('"%s"' % s for s in spectra))
+    (cfg / "spooky.cfg").write_text(text)
+    out = str(tmp_path / "run")
+    names = ["vx", "vy", "vz", "bx", "by", "bz"]
+    ic = rc.mri_eigenmode(0.0)
+    write_ic(out, ic)
+    run(str(cfg), out, "-r", "0")
+    snaps = read_all(out, names, (64, 64, 64))
+    assert snaps[-1][0]["t_save"] == pytest.approx(1.0, abs=1e-12)
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.mri_eigenmode(hdr["t_save"]), names)
+        assert err < 1e-7, (hdr, err)
+
+    lines = open(os.path.join(out, "data", "spectrum.spooky")).read().splitlines(True)
+    assert lines[0].startswith("## This file contains the 1d (shell-integrated) energy spectral densities")
+    assert lines[1].lstrip("##").rstrip("\n").split("\t")[1:-1] == spectra
+    n, L = (64, 64, 64), (1.0, 1.5, 1.0)
+    kmax = np.sqrt(sum((2.0 / 3.0 * rc.TWO_PI / Li * (ni // 2 - 1)) ** 2 for ni, Li in zip(n, L)))
+    nb = int(kmax / (rc.TWO_PI) + 0.5)
+    assert lines[2].startswith("wavevector")
+    kvec = np.array(lines[2].rstrip("\n").split("\t")[1:-1], dtype="float64")
+    assert np.allclose(kvec, rc.TWO_PI * np.arange(nb), rtol=1e-8)
+    rows = {}
+    for ln in lines[3:]:
+        sp = ln.rstrip("\n").split("\t")
+        assert sp[0] == "t" and sp[-1] == ""
+        rows.setdefault(sp[2], []).append((float(sp[1]), np.array(sp[3:-1], dtype="float64")))
+    # Eth needs the heat modules: no line, as in the reference
+    assert sorted(rows) == sorted(s for s in spectra if s != "Eth")
+    times = [t for t, _ in rows["Kx"]]
+    # one row per timevar_step crossing (0.1, check_output.cu:31) from t = 0
+    assert len(times) >= 10 and times[0] == 0.0 and np.all(np.diff(times) > 0) and times[-1] <= 1.0 + 1e-12
+    c = {k: np.fft.rfftn(ic[k]) for k in names}
+    # t = 0: the single mode (kx, kz) = (2 pi, 4 pi) sits in bin round(sqrt 5) = 2
+    # with <vx^2> = a^2 / 2
+    kx0 = rows["Kx"][0][1]
+    assert kx0.shape == (nb,) and kx0[2] == pytest.approx(0.5e-8, rel=1e-7)
+    assert np.abs(np.delete(kx0, 2)).max() < 1e-20
+    for name, (a, b) in (("Kx", ("vx", "vx")), ("bxbz", ("bx", "by")), ("Mz", ("bz", "bz"))):
+        ref = shell_box(c[a], c[b], n, L, nb)
+        got = rows[name][0][1]
+        assert np.abs(got - ref).max() <= 1e-7 * np.abs(ref).max(), name  # 8 significant digits in the file
+    # lorentz_y and emfpower_y carry the same energy transfer (their bin sums agree)
+    for (t1, l), (t2, e) in zip(rows["lorentz_y"], rows["emfpower_y"]):
+        assert t1 == t2 and l.sum() == pytest.approx(e.sum(), rel=1e-6, abs=1e-30)
+    user = open(os.path.join(out, "data", "user-timevar.spooky")).read().splitlines()
+    assert user[1].split() == ["##", "kpartheta", "uservar1"]
+    assert len(user) == 2 + len(times)
+    for ln in user[2:]:
+        assert ln.split() == ["-1.00000000e+00", "0.00000000e+00"]  # kpartheta needs anisotropic conduction
