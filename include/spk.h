@@ -132,6 +132,18 @@ int spk_main_loop(spk_solver s, int nsteps);
  * autotune run at spk_solver_create found it faster than the unfused kernels
  * on this grid; 2: always where allowed.  All give the same results to rounding. */
 int spk_solver_set_fused(spk_solver s, int mode);
+/* RKL2 cycles planned on the device (on = 1): the stage count and coefficients
+ * of RKLegendre::compute_cycle_RKL2 (rkl.cu:466-585) are computed by a one-thread
+ * kernel from the step's dt and dt_sts, and the cycle launches the stage count
+ * of the sts cap (dt = safety_sts dt_sts, compute_dt.cu:274-282) with the stages
+ * beyond the step's count returning at once.  No host synchronization per cycle,
+ * so steps can be captured in a graph.  Needs super-time-stepping of an
+ * incompressible run and at most 64 stages at the cap.  Same results as the
+ * host-planned cycle (on = 0, the default). */
+int spk_solver_set_rkl_device(spk_solver s, int on);
+/* The last device plan: its stage count, whether the step's count exceeded the
+ * plan (overflow = 1: the cycle was cut short), and the stages launched per cycle. */
+int spk_solver_rkl_plan(spk_solver s, int *stages, int *overflow, int *smax);
 /* Which back end spk_main_loop will run: *fused = 1 for the fused kernel;
  * *points = points per thread of its x line (the autotuned tile). */
 int spk_solver_pipeline(spk_solver s, int *fused, int *points);
@@ -189,6 +201,33 @@ int spk_user_timevar(spk_solver s, const char *name, double *out);
  * (unshear_output.cu:21-54), which first multiplies a copy of every field by
  * exp(-i ky Lx S fmod(t, 2 Ly / (S Lx)) / 2).  Synchronizes to read t. */
 int spk_snapshot_real(spk_solver s);
+
+/* ---- batched ensemble: many independent boxes advanced together ----
+ * The reference runs one box per process (Supervisor, supervisor.cu:20-213);
+ * an ensemble holds nboxes solvers (box i: params[i], any grid and physics the
+ * solver supports except shearing boxes, all in precision bulk) on device
+ * `device`, spread over `lanes` HIP streams (box i on lane i % lanes; 0: 4).
+ * Boxes with super-time-stepping plan their RKL2 cycles on the device. */
+typedef struct spk_ensemble_s *spk_ensemble;
+int spk_ensemble_create(int device, int nboxes, const spk_params *params, int bulk, int lanes, spk_ensemble *out);
+int spk_ensemble_destroy(spk_ensemble e);
+int spk_ensemble_size(spk_ensemble e, int *nboxes, int *lanes);
+/* The solver of box i, for initial conditions, time state and diagnostics
+ * (calls on it are ordered after the ensemble's queued steps). */
+int spk_ensemble_box(spk_ensemble e, int i, spk_solver *out);
+/* Every box advances nsteps (even) steps of the main loop (spk_main_loop) with
+ * its own dt.  With the graph on (default) two steps of every box are captured
+ * once into one HIP graph and each pair of steps is one graph launch; the graph
+ * is captured again when a box was stepped outside it.  Asynchronous. */
+int spk_ensemble_run(spk_ensemble e, int nsteps);
+int spk_ensemble_sync(spk_ensemble e);
+/* on = 0: the same steps launched box by box into the lanes, without a graph */
+int spk_ensemble_set_graph(spk_ensemble e, int on);
+int spk_ensemble_set_lanes(spk_ensemble e, int lanes);
+/* Lane-count autotune over 1, 2, 4, 8, 16 lanes: each candidate times `probe`
+ * (even) steps of every box after a warm-up pair; every box's state and clock
+ * are saved before and restored after each candidate.  Keeps the fastest. */
+int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,17 @@ SIGNATURES = {
     "spk_spectrum_named": [_VP, ctypes.c_char_p, _VP, _I, _VP],
     "spk_user_timevar": [_VP, ctypes.c_char_p, _VP],
     "spk_snapshot_real": [_VP],
+    "spk_solver_set_rkl_device": [_VP, _I],
+    "spk_solver_rkl_plan": [_VP, _VP, _VP, _VP],
+    "spk_ensemble_create": [_I, _I, _VP, _I, _I, _VP],
+    "spk_ensemble_destroy": [_VP],
+    "spk_ensemble_size": [_VP, _VP, _VP],
+    "spk_ensemble_box": [_VP, _I, _VP],
+    "spk_ensemble_run": [_VP, _I],
+    "spk_ensemble_sync": [_VP],
+    "spk_ensemble_set_graph": [_VP, _I],
+    "spk_ensemble_set_lanes": [_VP, _I],
+    "spk_ensemble_tune": [_VP, _I, _VP, _VP],
 }
 
 
@@ -241,12 +252,16 @@ VAR_NAMES = ["VX", "VY", "VZ", "BX", "BY", "BZ", "TH", "VEL", "MAG", "NUM_FIELDS
 class Solver:
     """Device state and time stepping of one run (Fields + TimeStepping + RKLegendre)."""
 
-    def __init__(self, ctx, precision="f64", **params):
+    def __init__(self, ctx, precision="f64", _handle=None, _params=None, **params):
         self.ctx = ctx
-        self.params = make_params(**params)
+        self.params = _params if _params is not None else make_params(**params)
         self.prec = _PREC[precision]
-        h = ctypes.c_void_p()
-        check(lib().spk_solver_create(ctx.h, ctypes.byref(self.params), self.prec, ctypes.byref(h)))
+        self.owned = _handle is None
+        if _handle is None:
+            h = ctypes.c_void_p()
+            check(lib().spk_solver_create(ctx.h, ctypes.byref(self.params), self.prec, ctypes.byref(h)))
+        else:
+            h = _handle
         self.h = h
         v = (ctypes.c_int * 10)()
         ntc = ctypes.c_size_t()
@@ -261,9 +276,9 @@ class Solver:
         self.d_fields, self.d_dfields, self.d_realbuf = f, d, r
 
     def destroy(self):
-        if self.h:
+        if self.h and self.owned:
             lib().spk_solver_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -397,9 +412,91 @@ class Solver:
         check(lib().spk_user_timevar(self.h, name.encode(), ctypes.byref(o)))
         return o.value
 
+    def set_rkl_device(self, on=True):
+        """RKL2 cycles planned on the device (no host synchronization per cycle)."""
+        check(lib().spk_solver_set_rkl_device(self.h, 1 if on else 0))
+
+    def rkl_plan(self):
+        """(stages of the last device plan, overflow flag, stages launched per cycle)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().spk_solver_rkl_plan(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     def snapshot_real(self):
         """The real fields a snapshot holds into the real buffer (unsheared in a shearing box)."""
         check(lib().spk_snapshot_real(self.h))
+
+
+class Ensemble:
+    """Many independent boxes advanced together (spk_ensemble): one solver per
+    box, boxes spread over HIP streams, two steps of every box per graph launch.
+    boxes: a list of parameter dicts (as Solver's keyword arguments)."""
+
+    def __init__(self, boxes, precision="f32", lanes=0, device=0):
+        self.params = [make_params(**b) for b in boxes]
+        arr = (SpkParams * len(self.params))(*self.params)
+        self.prec = _PREC[precision]
+        h = ctypes.c_void_p()
+        check(lib().spk_ensemble_create(device, len(self.params), arr, self.prec, lanes, ctypes.byref(h)))
+        self.h = h
+        # host transfers go through the null stream after an ensemble sync
+        self.ctx = Ctx(device)
+        self.boxes = []
+        for i, p in enumerate(self.params):
+            b = ctypes.c_void_p()
+            check(lib().spk_ensemble_box(h, i, ctypes.byref(b)))
+            self.boxes.append(Solver(self.ctx, precision, _handle=b, _params=p))
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def lanes(self):
+        n, l = ctypes.c_int(), ctypes.c_int()
+        check(lib().spk_ensemble_size(self.h, ctypes.byref(n), ctypes.byref(l)))
+        return l.value
+
+    def run(self, nsteps):
+        check(lib().spk_ensemble_run(self.h, nsteps))
+
+    def sync(self):
+        check(lib().spk_ensemble_sync(self.h))
+
+    def set_graph(self, on):
+        check(lib().spk_ensemble_set_graph(self.h, 1 if on else 0))
+
+    def set_lanes(self, lanes):
+        check(lib().spk_ensemble_set_lanes(self.h, lanes))
+
+    def tune(self, probe=2):
+        best, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().spk_ensemble_tune(self.h, probe, ctypes.byref(best), ctypes.byref(ms)))
+        return best.value, ms.value
+
+    def set_fields(self, i, c):
+        self.sync()
+        self.boxes[i].set_fields(c)
+
+    def get_fields(self, i):
+        self.sync()
+        return self.boxes[i].get_fields()
+
+    def time(self, i):
+        self.sync()
+        return self.boxes[i].time()
+
+    def destroy(self):
+        if self.h:
+            for b in self.boxes:
+                b.h = None
+            lib().spk_ensemble_destroy(self.h)
+            self.h = None
+            self.ctx.close()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def padded_real(r):

@@ -29,7 +29,9 @@ namespace spk {
 template <class T, int N, int DIR, int AXIS>
 __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_pass_strided(
     const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
-    int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale) {
+    int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale,
+    const int *__restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
     constexpr int TPL = N / E;
     constexpr int TC = strided_tcols(N);
@@ -179,7 +181,7 @@ static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride
         dim3 grid((unsigned)ntiles * nouter, a.nfields);                                          \
         k_pass_strided<T, NN, DIR, AXIS><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
             (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, ntiles, a.fstride_in, \
-            a.fstride_out, twp, (T)a.scale);                                                       \
+            a.fstride_out, twp, (T)a.scale, ctx->gate);                                            \
         break;                                                                                     \
     }
     switch (n) {

@@ -13,6 +13,10 @@
 struct spk_ctx_s {
     int device;
     hipStream_t stream;
+    // When set, transform kernels launched on this context return at once unless
+    // *gate is non-zero (device-sized loops: the stages of an RKL2 cycle beyond
+    // the stage count the device computed for this step)
+    const int *gate = nullptr;
 };
 
 namespace spk {
@@ -106,5 +110,10 @@ int launch_pass_z_r2c(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, 
                       const PassArgs &a);
 int launch_pass_z_c2r(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout,
                       const PassArgs &a);
+
+// solver internals the ensemble needs (spk_solver.hip)
+spk_ctx solver_ctx(spk_solver s);
+int solver_host_step(spk_solver s);
+void solver_advance_host(spk_solver s, int n);
 
 }  // namespace spk

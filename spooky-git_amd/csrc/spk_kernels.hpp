@@ -51,7 +51,8 @@ struct InvProg {
 template <class T, int N>
 __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_inv_x(
     const cpx<double> *__restrict__ U, cpx<T> *__restrict__ W, long long fs, InvProg prog, KGeom kg,
-    double scale, const cpx<T> *__restrict__ tw) {
+    double scale, const cpx<T> *__restrict__ tw, const int *__restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
     constexpr int TPL = N / E;
     constexpr int TC = strided_tcols(N);
@@ -193,7 +194,8 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cons
                                               cpx<T> *__restrict__ P, long long nrows, long long fs,
                                               const cpx<T> *__restrict__ tw_nz,
                                               const cpx<T> *__restrict__ tw_m, ZConst zc,
-                                              double *__restrict__ pmax) {
+                                              double *__restrict__ pmax, const int *__restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     using Z = ZProg<PROG>;
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;

@@ -180,11 +180,72 @@ struct RklConst {
     double nu, nu_m, nu_th;
 };
 
+// Device-sized RKL2 cycle: the stage count and coefficient table of the
+// recurrence (rkl.cu:466-585) computed on the device from the step's dt and
+// dt_sts, so that a cycle needs no host synchronization and can be captured in
+// a graph.  RKL_SMAX stages bound what a plan can hold.
+constexpr int RKL_SMAX = 64;
+struct RklPlan {
+    int s, overflow;
+    int active[RKL_SMAX + 1];   // active[j] = 1 for stages 1 <= j <= s
+    double coef[4 * RKL_SMAX];  // mu, nu, mu~, gamma~ of stage j at 4 (j - 1)
+};
+
+// The recurrence of rkl2_coeffs (host), operation for operation and without
+// contraction into FMAs, so that both give the same doubles.
+__global__ void k_rkl2_plan(const TimeState *__restrict__ ts, int smax, RklPlan *__restrict__ plan) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    const double scrh = ts->dt / ts->dt_sts;
+    const double s_str = 4.0 * (1.0 + 2.0 * scrh) / (1.0 + sqrt(9.0 + 16.0 * scrh));
+    // 1 + (int)s_str as on the host; past smax (or NaN) the plan is cut at smax
+    int s_RKL = s_str < (double)smax ? 1 + (int)(s_str) : smax + 1;
+    plan->overflow = s_RKL > smax;
+    if (s_RKL > smax) s_RKL = smax;
+    plan->s = s_RKL;
+    const double w1 = 4.0 / (s_RKL * s_RKL + s_RKL - 2.0);
+    double mu_tilde_j = w1 / 3.0;
+    double b_j, b_jm1, b_jm2, a_jm1;
+    b_j = b_jm1 = b_jm2 = 1.0 / 3.0;
+    a_jm1 = 1.0 - b_jm1;
+    for (int j = 0; j < 4 * smax; j++) plan->coef[j] = 0.0;
+    plan->coef[2] = mu_tilde_j;
+    for (int stage = 2; stage <= s_RKL; stage++) {
+        const double mu_j = (2. * stage - 1.) / stage * b_j / b_jm1;
+        mu_tilde_j = w1 * mu_j;
+        const double gamma_j = -a_jm1 * mu_tilde_j;
+        const double nu_j = -(stage - 1.) * b_j / (stage * b_jm2);
+        double *c = plan->coef + 4 * (stage - 1);
+        c[0] = mu_j;
+        c[1] = nu_j;
+        c[2] = mu_tilde_j;
+        c[3] = gamma_j;
+        b_jm2 = b_jm1;
+        b_jm1 = b_j;
+        a_jm1 = 1.0 - b_jm1;
+        b_j = 0.5 * (stage * stage + 3.0 * stage) / (stage * stage + 3.0 * stage + 2);
+    }
+    plan->active[0] = 0;
+    for (int j = 1; j <= smax; j++) plan->active[j] = j <= s_RKL;
+}
+
+// plan == nullptr: the host passes the stage's coefficients; otherwise they come
+// from the device plan and the stage returns at once beyond its stage count
 template <class T>
 __global__ __launch_bounds__(256) void k_rkl2(const cpx<T> *__restrict__ Q, cpx<double> *__restrict__ U,
                                               cpx<double> *__restrict__ Uc0, cpx<double> *__restrict__ Uc1,
                                               cpx<double> *__restrict__ dU0, RklConst c, int stage, double mu,
-                                              double nu, double dtmut, double gdt) {
+                                              double nu, double dtmut, double gdt, const RklPlan *__restrict__ plan,
+                                              const TimeState *__restrict__ ts) {
+    if (plan != nullptr) {
+        if (!plan->active[stage]) return;
+        const double *cf = plan->coef + 4 * (stage - 1);
+        const double dt = ts->dt;
+        mu = cf[0];
+        nu = cf[1];
+        dtmut = cf[2] * dt;
+        gdt = cf[3] * dt;
+    }
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= c.ntc) return;
     const long long N = c.ntc;
@@ -600,6 +661,9 @@ struct spk_solver_s {
     void *Breal;        // real-space B rows (padded, bulk precision), made once per RKL2 cycle
     cpx<double> *diag;  // 6 FP64 fields of scratch for the timevar diagnostics, allocated at first use
     int rkl_cache_b;    // B is not an sts variable: its real-space rows stay valid for a whole cycle
+    int rkl_device;     // RKL2 cycles planned on the device (no host synchronization per cycle)
+    int rkl_smax;       // stage count at the sts cap: the stages a device-planned cycle launches
+    RklPlan *rkl_plan;  // device
     InvProg inv_rkl_b, inv_rkl_g;
     cpx<double> *rUc0, *rUc1, *rdU0;
     TimeState *ts;
@@ -643,7 +707,8 @@ int launch_inv_x(spk_solver s, const InvProg &prog, int nslots, void *W, double 
         constexpr int TC = strided_tcols(NN);                                                     \
         dim3 grid((unsigned)(((long long)g.ny * g.nzh + TC - 1) / TC), nslots);                  \
         k_inv_x<T, NN><<<grid, TC * strided_tpl(NN), 0, s->ctx->stream>>>(s->U, (cpx<T> *)W,     \
-                                                                           g.ntc, prog, kg, scale, tw); \
+                                                                           g.ntc, prog, kg, scale, tw, \
+                                                                           s->ctx->gate);              \
         break;                                                                                    \
     }
     switch (g.nx) {
@@ -671,18 +736,18 @@ int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax,
     cpx<T> *pp = (cpx<T> *)P;
     hipStream_t st = s->ctx->stream;
     switch (prog) {
-        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
-        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
-        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
+        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
+        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
         case ZP_U | ZP_B | ZP_TH:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
             break;
         case ZP_U | ZP_B | ZP_TH | ZP_G:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
             break;
-        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax); break;
+        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
         case ZP_B | ZP_G | ZP_BR:
-            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax);
+            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
             break;
         default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
     }
@@ -974,7 +1039,7 @@ int rkl2_stage_launch(spk_solver s, int stage, double mu, double nu, double dtmu
     RklConst c = s->rkl;
     c.kg = kgeom(s, s->tremap * s->p.shear);
     k_rkl2<T><<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>((const cpx<T> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c,
-                                                           stage, mu, nu, dtmut, gdt);
+                                                           stage, mu, nu, dtmut, gdt, nullptr, nullptr);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
@@ -1045,12 +1110,42 @@ int rkl2_cycle(spk_solver s) {
     return SPK_OK;
 }
 
+// rkl2_cycle with the stage count and coefficients planned on the device:
+// RKL_SMAX-bounded stages are launched (rkl_smax, the stage count at the sts cap
+// dt = safety_sts dt_sts) and those beyond the step's count return at once.
+int rkl2_cycle_device(spk_solver s) {
+    if (s->nsts == 0) return SPK_OK;
+    hipStream_t st = s->ctx->stream;
+    k_rkl2_plan<<<1, 64, 0, st>>>(s->ts, s->rkl_smax, s->rkl_plan);
+    SPK_LAUNCH_CHECK();
+    if (s->rkl_aniso && s->rkl_cache_b) SPK_TRY(rkl2_b_real(s));
+    RklConst c = s->rkl;
+    c.kg = kgeom(s, s->tremap * s->p.shear);
+    for (int stage = 1; stage <= s->rkl_smax; stage++) {
+        if (s->rkl_aniso) {
+            s->ctx->gate = &s->rkl_plan->active[stage];  // a device address; never read on the host
+            const int rc = s->rkl_cache_b ? rkl2_front_cached(s) : rkl2_front(s);
+            s->ctx->gate = nullptr;
+            SPK_TRY(rc);
+        }
+        if (s->bulk == SPK_F64)
+            k_rkl2<double><<<nblk(s->g.ntc), 256, 0, st>>>((const cpx<double> *)s->P2, s->U, s->rUc0, s->rUc1,
+                                                           s->rdU0, c, stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts);
+        else
+            k_rkl2<float><<<nblk(s->g.ntc), 256, 0, st>>>((const cpx<float> *)s->P2, s->U, s->rUc0, s->rUc1,
+                                                          s->rdU0, c, stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts);
+        SPK_LAUNCH_CHECK();
+    }
+    return SPK_OK;
+}
+
 int hydro_mhd_advance(spk_solver s) {
     const bool sts = s->p.supertimestepping && s->nsts > 0;
+    auto cycle = [&]() { return s->rkl_device ? rkl2_cycle_device(s) : rkl2_cycle(s); };
     SPK_TRY(compute_dt(s));
-    if (sts && (s->step % 2) == 1) SPK_TRY(rkl2_cycle(s));
+    if (sts && (s->step % 2) == 1) SPK_TRY(cycle());
     SPK_TRY(rk3(s));
-    if (sts && (s->step % 2) == 0) SPK_TRY(rkl2_cycle(s));
+    if (sts && (s->step % 2) == 0) SPK_TRY(cycle());
     s->step += 1;
     return clock(s, 0, 1, 0, 0, 0.0);
 }
@@ -1143,8 +1238,8 @@ int step_fused(spk_solver s, bool w_valid) {
 }
 
 void free_all(spk_solver s) {
-    void *bufs[] = {s->U, s->S, s->dU, s->realbuf, s->W, s->P, s->W2, s->P2, s->Breal, s->diag, s->rUc0, s->rUc1, s->rdU0,
-                    s->ts, s->pmax, s->part, s->res, s->spec};
+    void *bufs[] = {s->U,    s->S,    s->dU,   s->realbuf, s->W,        s->P,  s->W2,   s->P2,   s->Breal, s->diag,
+                    s->rUc0, s->rUc1, s->rdU0, s->rkl_plan, s->ts, s->pmax, s->part, s->res,  s->spec};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     twiddles_destroy(&s->twb);
@@ -1293,6 +1388,12 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
         SPK_TRY(dalloc(&s->rUc0, f64 * s->nsts));
         SPK_TRY(dalloc(&s->rUc1, f64 * s->nsts));
         SPK_TRY(dalloc(&s->rdU0, f64 * s->nsts));
+        SPK_TRY(dalloc(&s->rkl_plan, sizeof(RklPlan)));
+        // stage count at r = dt / dt_sts = safety_sts (the cap of compute_dt), with a
+        // margin for the rounding of that quotient
+        const double r = (double)p.safety_sts * (1.0 + 1e-12);
+        s->rkl_smax = 1 + (int)(4.0 * (1.0 + 2.0 * r) / (1.0 + sqrt(9.0 + 16.0 * r)));
+        SPK_HIP(hipMemsetAsync(s->rkl_plan, 0, sizeof(RklPlan), ctx->stream));
     }
     SPK_TRY(dalloc(&s->ts, sizeof(TimeState)));
     const long long rows = (long long)p.nx * p.ny;
@@ -1981,6 +2082,45 @@ extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int ca
     SPK_HIP(hipStreamSynchronize(st));
     s->products_valid = false;
     for (int k = 0; k < NK && k < cap; k++) out_ms[k] = ms[k];
+    return SPK_OK;
+}
+
+namespace spk {
+spk_ctx solver_ctx(spk_solver s) { return s->ctx; }
+int solver_host_step(spk_solver s) { return s->step; }
+// host-side step count after n steps that ran from a replayed graph
+void solver_advance_host(spk_solver s, int n) {
+    s->step += n;
+    s->products_valid = false;
+}
+}  // namespace spk
+
+extern "C" int spk_solver_set_rkl_device(spk_solver s, int on) {
+    SPK_CHECK_SOLVER(s);
+    if (!on) {
+        s->rkl_device = 0;
+        return SPK_OK;
+    }
+    const spk_params &p = s->p;
+    SPK_REQUIRE(s->nsts > 0 && p.incompressible && !p.heat_equation, SPK_ERR_ARG,
+                "spk_solver_set_rkl_device: needs super-time-stepping of an incompressible run (the sts cap of "
+                "compute_dt bounds the stage count only there)");
+    SPK_REQUIRE(s->rkl_smax <= RKL_SMAX, SPK_ERR_ARG,
+                "spk_solver_set_rkl_device: safety_sts = %d allows %d stages; a device plan holds %d", p.safety_sts,
+                s->rkl_smax, RKL_SMAX);
+    s->rkl_device = 1;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_rkl_plan(spk_solver s, int *stages, int *overflow, int *smax) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(s->rkl_plan, SPK_ERR_ARG, "spk_solver_rkl_plan: no super-time-stepping in this run");
+    int h[2];
+    SPK_HIP(hipMemcpyAsync(h, s->rkl_plan, sizeof(h), hipMemcpyDeviceToHost, s->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (stages) *stages = h[0];
+    if (overflow) *overflow = h[1];
+    if (smax) *smax = s->rkl_smax;
     return SPK_OK;
 }
 

@@ -119,3 +119,32 @@ def mti_growth_rate(kx, chi, nu, eta, b0, N2):
     r = np.roots(coeff)
     real = np.real(r[np.abs(np.imag(r)) < 1e-9 * np.maximum(1.0, np.abs(r))])
     return float(np.max(real)) if real.size else float("nan")
+
+
+def hbi_noise(shape, L=(1.0, 1.0, 1.0), seed=0, amp=0.01, b0=1e-5):
+    """State [7, ...] of the 2D heat-flux-buoyancy problem (problems/hbi2d,
+    its initial-condition script): u_x, u_y with uniform random amplitudes
+    amp * U(-1/2, 1/2) * sqrt(27/8 N) and uniform random phases at every
+    scale, u_z = 0, B = (0, b0, 0), theta = 0."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    csh = (nx, ny, nz // 2 + 1)
+    fact = np.sqrt(27.0 / 8.0 * nx * ny * nz)
+
+    def noise():
+        c = amp * rng.uniform(-0.5, 0.5, csh) * np.exp(2j * np.pi * rng.uniform(-0.5, 0.5, csh)) * fact
+        return np.fft.rfftn(np.fft.irfftn(c, s=shape, axes=(0, 1, 2)))  # Hermitian-consistent
+
+    out = np.zeros((7,) + csh, dtype=np.complex128)
+    out[0], out[1] = noise(), noise()
+    out[4, 0, 0, 0] = b0 * nx * ny * nz
+    return out
+
+
+def hbi_params(n=512, nu=1.0 / 12800.0, eta=1.0 / 12800.0, chi=1.0 / 1600.0, N2=0.2, OmegaT2=-1.0):
+    """Solver parameters of problems/hbi2d/spooky.cfg (gravity and stratification
+    along y, B along y, RKL2 on theta), on an n x n x 2 box of unit size."""
+    return dict(nx=n, ny=n, nz=2, incompressible=1, boussinesq=1, stratification=1, strat_direction=1, mhd=1,
+                anisotropic_diffusion=1, supertimestepping=1, sts_mask=1 << 6, safety_sts=100, nu=nu, nu_m=eta,
+                nu_th=1e-6, reynolds_ani=1.0 / chi, N2=N2, OmegaT2=OmegaT2, cfl=1.5, cfl_par=1.0, cfl_sts=1.0,
+                safety_source=0.2, lx=1.0, ly=1.0, lz=1.0, t_final=1e9, toutput_flow=1e9)

@@ -1,0 +1,172 @@
+"""Batched ensemble (spk_ensemble) and device-planned RKL2 cycles on the GPU.
+
+- A device-planned RKL2 cycle (the recurrence of rkl.cu:466-585 evaluated by a
+  one-thread kernel, extra stages returning at once) gives the host-planned
+  cycle's state, and its stage count is the recurrence's.
+- Every box of an ensemble (uneven grids and physics, several lanes, two steps
+  per graph replay) ends where the same solver run on its own ends, with the
+  graph and without it, and where the FP64 oracle ends.
+- The lane autotune leaves every box's state and clock as it found them.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from physics_cases import CASES, initial_state, relerr
+from spkmod import spk
+
+pytestmark = pytest.mark.gpu
+ini = __import__("importlib").import_module("spooky-git_amd.initial")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = spk.Ctx(0)
+    yield c
+    c.close()
+
+
+def rkl_stages(r):
+    """Stage count of rkl.cu:473-475 for r = dt / dt_sts."""
+    return 1 + int(4.0 * (1.0 + 2.0 * r) / (1.0 + np.sqrt(9.0 + 16.0 * r)))
+
+
+def test_device_rkl_plan_matches_host(ctx):
+    name = "mhd_bouss_rkl2"
+    p = CASES[name]["params"]
+    u0 = initial_state(name)
+    a = spk.Solver(ctx, "f64", **p)
+    b = spk.Solver(ctx, "f64", **p)
+    b.set_rkl_device(True)
+    for s in (a, b):
+        s.set_fields(u0)
+    for _ in range(3):
+        a.main_loop(2)
+        b.main_loop(2)
+        fa, fb = a.get_fields(), b.get_fields()
+        assert np.max(np.abs(fa - fb)) <= 1e-14 * np.max(np.abs(fa))
+    t = b.time()
+    stages, overflow, smax = b.rkl_plan()
+    assert overflow == 0 and smax == rkl_stages(p["safety_sts"] * (1 + 1e-12)) == 9
+    assert stages == rkl_stages(t["dt"] / t["dt_sts"]) and stages > 2
+    a.destroy()
+    b.destroy()
+    # refused where the stage count has no bound
+    h = spk.Solver(ctx, "f64", **CASES["heat"]["params"])
+    with pytest.raises(RuntimeError):
+        h.set_rkl_device(True)
+    h.destroy()
+
+
+def boxes():
+    """Uneven boxes: MHD 16^3 and 32x16x16 with different coefficients, and two
+    32x32x2 Boussinesq-MHD boxes with anisotropic conduction and RKL2 on theta."""
+    T = 2 * np.pi
+    big = dict(t_final=1e9, toutput_flow=1e9)
+    mhd = dict(incompressible=1, mhd=1, lx=T, ly=T, lz=T, cfl=0.5, **big)
+    out = [dict(mhd, nx=16, ny=16, nz=16, nu=3e-3, nu_m=1e-2),
+           dict(mhd, nx=32, ny=16, nz=16, nu=1e-2, nu_m=2e-3),
+           dict(mhd, nx=16, ny=16, nz=16, nu=5e-3, nu_m=5e-3),
+           ini.hbi_params(32, nu=2e-3, eta=1e-3, chi=0.05),
+           ini.hbi_params(32, nu=1e-3, eta=3e-3, chi=0.02)]
+    return out
+
+
+def box_state(p, seed):
+    shape = (p["nx"], p["ny"], p["nz"])
+    L = (p["lx"], p["ly"], p["lz"])
+    if p.get("boussinesq"):
+        st = ini.hbi_noise(shape, L, seed=seed, amp=0.5, b0=1.0)
+        st[6] = 0.1 * np.fft.rfftn(np.random.default_rng(seed + 1).standard_normal(shape)) * ini.dealias_mask(
+            *shape, *L)
+        return st
+    return np.concatenate([0.5 * ini.random_solenoidal(shape, L, seed=seed, k0=2.0),
+                           0.5 * ini.random_solenoidal(shape, L, seed=seed + 7, k0=2.0)])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_matches_standalone_and_oracle(ctx, graph):
+    bx = boxes()
+    states = [box_state(p, 100 + i) for i, p in enumerate(bx)]
+    ens = spk.Ensemble(bx, "f64", lanes=2)
+    ens.set_graph(graph)
+    for i, st in enumerate(states):
+        ens.set_fields(i, st)
+    ens.run(6)
+    got = [ens.get_fields(i) for i in range(len(bx))]
+    times = [ens.time(i) for i in range(len(bx))]
+    with pytest.raises(RuntimeError):
+        ens.run(3)  # a replay advances every box by two steps
+    ens.destroy()
+    for i, (p, st) in enumerate(zip(bx, states)):
+        s = spk.Solver(ctx, "f64", **p)
+        if p.get("supertimestepping"):
+            s.set_rkl_device(True)
+        s.set_fields(st)
+        for _ in range(3):
+            s.main_loop(2)
+        ref = s.get_fields()
+        assert np.max(np.abs(got[i] - ref)) <= 1e-14 * np.max(np.abs(ref)), i
+        assert times[i]["step"] == 6 and times[i]["t"] == s.time()["t"]
+        s.destroy()
+        sim = orc.Sim(**p)
+        sim.set_fields(st)
+        for _ in range(6):
+            sim.main_loop(1)
+        assert relerr(got[i], sim.fields) < 1e-10, (i, relerr(got[i], sim.fields))
+
+
+def test_ensemble_fp32_and_recapture(ctx):
+    """FP32 bulk within the solver's FP32 tolerance of the oracle; a box stepped
+    outside the graph (parity change) makes the next run capture again."""
+    bx = boxes()
+    states = [box_state(p, 200 + i) for i, p in enumerate(bx)]
+    ens = spk.Ensemble(bx, "f32", lanes=4)
+    for i, st in enumerate(states):
+        ens.set_fields(i, st)
+    ens.run(2)
+    ens.sync()
+    ens.boxes[0].main_loop(1)
+    ens.run(2)
+    got = [ens.get_fields(i) for i in range(len(bx))]
+    assert ens.time(0)["step"] == 5 and all(ens.time(i)["step"] == 4 for i in range(1, len(bx)))
+    ens.destroy()
+    for i, (p, st) in enumerate(zip(bx, states)):
+        sim = orc.Sim(**p)
+        sim.set_fields(st)
+        for _ in range(5 if i == 0 else 4):
+            sim.main_loop(1)
+        assert relerr(got[i], sim.fields) < 2e-4, (i, relerr(got[i], sim.fields))
+
+
+def test_ensemble_tune_restores_state(ctx):
+    bx = boxes()
+    states = [box_state(p, 300 + i) for i, p in enumerate(bx)]
+    ens = spk.Ensemble(bx, "f32")
+    for i, st in enumerate(states):
+        ens.set_fields(i, st)
+    ens.run(2)
+    before = [ens.get_fields(i) for i in range(len(bx))]
+    tb = [ens.time(i) for i in range(len(bx))]
+    lanes, ms = ens.tune(probe=2)
+    assert lanes in (1, 2, 4) and ms > 0 and ens.lanes() == lanes
+    for i in range(len(bx)):
+        assert np.array_equal(ens.get_fields(i), before[i])
+        t = ens.time(i)
+        assert t["t"] == tb[i]["t"] and t["step"] == tb[i]["step"]
+    # and it keeps stepping from there as an untuned ensemble would
+    ens.run(2)
+    after = ens.get_fields(3)
+    ens.destroy()
+    ref = spk.Ensemble(bx, "f32", lanes=1)
+    for i, st in enumerate(states):
+        ref.set_fields(i, st)
+    ref.run(4)
+    assert np.max(np.abs(ref.get_fields(3) - after)) <= 1e-14 * np.max(np.abs(after))
+    ref.destroy()
+
+
+def test_ensemble_refuses_shearing_box(ctx):
+    p = dict(CASES["shear_mhd"]["params"])
+    with pytest.raises(RuntimeError):
+        spk.Ensemble([p], "f64")
