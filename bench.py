@@ -19,7 +19,11 @@ Beside the timed line (not part of `value`):
                              steps (energy, cross helicity, spectra);
   taylor_green_64            64^3 HD Taylor-Green, 100 steps, GPU and oracle;
   mti_rkl2                   256^3 Boussinesq MTI with anisotropic conduction,
-                             FP64 RKL2 on theta: steps/s and stages per cycle.
+                             FP64 RKL2 on theta: steps/s and stages per cycle;
+  ensemble                   48 MHD boxes at 64^3 + 16 at 128^3 + 32 Boussinesq-MHD
+                             boxes at 512x512x2 with RKL2 conduction, FP32 bulk,
+                             advanced together (spk_ensemble): steps of every box
+                             per second, and representative boxes against FP64.
 """
 import argparse
 import importlib
@@ -48,7 +52,8 @@ def parse():
     ap.add_argument("--mti-steps", type=int, default=10)
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the Taylor-Green 64^3 and MTI legs")
-    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti"], default="full",
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble"], default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
                          "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs); "
                          "mti: only the MTI leg (--mti-steps steps)")
@@ -290,6 +295,97 @@ def mti_leg(spk, ini, torch, ctx, n, steps, warmup, cpu):
     return res
 
 
+def ensemble_boxes(ini, seed=2026):
+    """The batched-ensemble workload: 48 MHD boxes at 64^3, 16 at 128^3 (2 pi
+    boxes, CFL 0.5) and 32 Boussinesq-MHD boxes at 512x512x2 with anisotropic
+    conduction and RKL2 on theta (problems/hbi2d physics); per box nu, eta
+    log-uniform in [1e-4, 1e-2] and chi_par uniform in [1e-3, 1e-1]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    boxes, kinds = [], []
+    for count, n in ((48, 64), (16, 128)):
+        for _ in range(count):
+            nu, eta = 10.0 ** rng.uniform(-4, -2, 2)
+            boxes.append(dict(mhd_params(n), nu=float(nu), nu_m=float(eta)))
+            kinds.append("mhd%d^3" % n)
+    for _ in range(32):
+        nu, eta = 10.0 ** rng.uniform(-4, -2, 2)
+        chi = rng.uniform(1e-3, 1e-1)
+        boxes.append(ini.hbi_params(512, nu=float(nu), eta=float(eta), chi=float(chi)))
+        kinds.append("hbi512x512x2")
+    return boxes, kinds
+
+
+def box_state(ini, p, seed):
+    shape = (p["nx"], p["ny"], p["nz"])
+    if p.get("boussinesq"):
+        return ini.hbi_noise(shape, seed=seed)
+    return ini.mhd_random(shape, seed=seed)
+
+
+def ensemble_leg(spk, ini, ctx, steps):
+    """The ensemble in FP32 bulk: lane count autotuned, then `steps` steps of
+    every box timed (host clock around run + sync) with the graph, launched box
+    by box without it, and box by box on one stream; representative boxes
+    against the FP64 path from the same initial state after the same steps."""
+    import numpy as np
+    boxes, kinds = ensemble_boxes(ini)
+    t0 = time.perf_counter()
+    ens = spk.Ensemble(boxes, "f32")
+    states = []
+    for i, p in enumerate(boxes):
+        states.append(box_state(ini, p, 5000 + i))
+        ens.set_fields(i, states[-1])
+    setup_s = time.perf_counter() - t0
+    print("# ensemble: %d boxes created and initialized in %.1f s" % (len(boxes), setup_s), file=sys.stderr,
+          flush=True)
+    lanes, _ = ens.tune(probe=2)
+    print("# ensemble: %d lanes after tuning" % lanes, file=sys.stderr, flush=True)
+
+    def timed(n):
+        ens.run(2)
+        ens.sync()
+        t1 = time.perf_counter()
+        ens.run(n)
+        ens.sync()
+        return (time.perf_counter() - t1) * 1000.0 / n
+
+    graph_ms = timed(steps)
+    ens.set_graph(False)
+    eager_ms = timed(steps)
+    ens.set_lanes(1)
+    one_stream_ms = timed(steps)
+    print("# ensemble: %.2f ms per step of every box (graph, %d lanes); %.2f without the graph; %.2f on one "
+          "stream" % (graph_ms, lanes, eager_ms, one_stream_ms), file=sys.stderr, flush=True)
+    ens.set_lanes(lanes)
+    ens.set_graph(True)
+    cells = sum(p["nx"] * p["ny"] * p["nz"] for p in boxes)
+    hbi = [i for i, k in enumerate(kinds) if k.startswith("hbi")]
+    stages = [rkl2_stages(ens.time(i)["dt"], ens.time(i)["dt_sts"]) for i in hbi]
+    acc = {}
+    for kind in dict.fromkeys(kinds):
+        i = kinds.index(kind)
+        nstep = int(ens.time(i)["step"])
+        got = ens.get_fields(i)
+        ref = spk.Solver(ctx, "f64", **boxes[i])
+        ref.set_fields(states[i])
+        ref.main_loop(nstep)
+        r = ref.get_fields()
+        e_ref = sum(ref.energy(f) for f in range(ref.nf))
+        e_got = sum(ens.boxes[i].energy(f) for f in range(ref.nf))
+        acc[kind] = dict(box=i, steps=nstep, rel_energy_err=abs(e_got - e_ref) / abs(e_ref),
+                         rel_l2_state=float(np.linalg.norm(got - r) / np.linalg.norm(r)))
+        ref.destroy()
+    ens.destroy()
+    return dict(boxes={k: kinds.count(k) for k in dict.fromkeys(kinds)}, bulk="f32", lanes=lanes, steps=steps,
+                ms_per_ensemble_step=round(graph_ms, 3), box_steps_per_s=round(len(boxes) * 1000.0 / graph_ms, 1),
+                cell_updates_per_s=cells * 1000.0 / graph_ms,
+                eager_lanes_ms_per_step=round(eager_ms, 3), one_stream_ms_per_step=round(one_stream_ms, 3),
+                rkl2_stages_hbi=[min(stages), max(stages)], setup_s=round(setup_s, 1),
+                accuracy_vs_f64=acc,
+                params="nu, eta log-uniform [1e-4, 1e-2]; chi_par uniform [1e-3, 1e-1] (hbi); seeds 5000+i")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -311,6 +407,9 @@ def main():
         return
     if a.mode == "mti":
         print(json.dumps(mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu)))
+        return
+    if a.mode == "ensemble":
+        print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps)))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
     if a.mode == "kernels":
@@ -363,6 +462,7 @@ def main():
     mti = mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu) if a.extra else None
     if mti:
         print("# mti %d^3: %s" % (n, json.dumps(mti)), file=sys.stderr, flush=True)
+    ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps) if a.extra and a.ensemble_steps > 0 else None
 
     cpu = None
     if not a.no_cpu:
@@ -397,6 +497,7 @@ def main():
                       if "f32" in res and "f64" in res else None),
         "taylor_green_64": tg,
         "mti_rkl2": mti,
+        "ensemble": ens,
         "fft_pass_gbs_f64": {k: round(v["gbs"], 1) for k, v in passes.items()},
         "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
                         for k, v in kern.items()},

@@ -26,15 +26,17 @@ namespace spk {
 
 // Line FFT over columns: element (f, o, n, c) at f*fstride + o*ostride + n*lstride + c.
 // AXIS (0 = x, 1 = y) only names the instantiation so profiles tell the passes apart.
-template <class T, int N, int DIR, int AXIS>
-__global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_pass_strided(
+// TC columns per block; the y pass of a quasi-2D grid (NZ = 2: two columns of
+// adjacent complex values per x) uses TC = 2, so that every lane has a column
+// and a wave reads 32 consecutive lines' two values as one contiguous run.
+template <class T, int N, int DIR, int AXIS, int TC = strided_tcols(N)>
+__global__ __launch_bounds__(TC * strided_tpl(N)) void k_pass_strided(
     const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
     int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale,
     const int *__restrict__ gate) {
     if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
     constexpr int TPL = N / E;
-    constexpr int TC = strided_tcols(N);
     constexpr int LL = lds_len(N);
     __shared__ cpx<T> lds[line_needs_lds<N>() ? TC * LL : 1];
 
@@ -177,6 +179,15 @@ static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride
 #define SPK_STRIDED_CASE(NN)                                                                       \
     case NN: {                                                                                     \
         constexpr int TC = strided_tcols(NN);                                                      \
+        if constexpr (AXIS == 1 && TC > 2) {                                                       \
+            if (ncols == 2) {                                                                      \
+                dim3 grid((unsigned)nouter, a.nfields);                                           \
+                k_pass_strided<T, NN, DIR, AXIS, 2><<<grid, 2 * strided_tpl(NN), 0, ctx->stream>>>(   \
+                    (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, 1, a.fstride_in, \
+                    a.fstride_out, twp, (T)a.scale, ctx->gate);                                    \
+                break;                                                                             \
+            }                                                                                      \
+        }                                                                                          \
         const int ntiles = (int)((ncols + TC - 1) / TC);                                           \
         dim3 grid((unsigned)ntiles * nouter, a.nfields);                                          \
         k_pass_strided<T, NN, DIR, AXIS><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
