@@ -229,6 +229,34 @@ int spk_ensemble_set_lanes(spk_ensemble e, int lanes);
  * are saved before and restored after each candidate.  Keeps the fastest. */
 int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, double *best_ms);
 
+/* ---- learned subgrid-stress closure (no counterpart in the reference) ----
+ * A 4-layer 3D convolution network on a periodic n^3 grid (n a multiple of 32):
+ * the filtered velocity (3 channels) -> 32 -> 32 -> 32 channels with 3^3 taps
+ * and GELU -> the 6 subgrid stresses tau_xx, tau_xy, tau_xz, tau_yy, tau_yz,
+ * tau_zz.  BF16 weights and activations on the matrix cores, FP32
+ * accumulation.  Parameters (FP32, spk_closure_num_params of them), layer by
+ * layer l = 1..4: W_l[27][32][cin_l] then b_l[32], cin = 16, 32, 32, 32; tap
+ * t = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) multiplies the input at (x + dx,
+ * y + dy, z + dz); input channels 3..15 of layer 1 and output channels 6..31
+ * of layer 4 are unused.  Velocities and stresses are device arrays in the
+ * padded real layout, FP64 [3 or 6][n][n][n + 2]; the network sees
+ * u * in_scale and predicts tau / out_scale[c]. */
+typedef struct spk_closure_s *spk_closure;
+size_t spk_closure_num_params(void);
+int spk_closure_create(spk_ctx ctx, int n, spk_closure *out); /* parameters start at 0 */
+int spk_closure_destroy(spk_closure c);
+int spk_closure_set_params(spk_closure c, const float *host_params); /* also resets Adam; synchronizes */
+int spk_closure_get_params(spk_closure c, float *host_params);       /* synchronizes */
+int spk_closure_set_scales(spk_closure c, double in_scale, const double *out_scale6);
+int spk_closure_forward(spk_closure c, const void *u, void *tau);
+/* gradient of the loss mean_{c < 6, points} (tau_pred / s_c - tau_true / s_c)^2
+ * with respect to every parameter (readable with spk_closure_get_grads); loss
+ * (optional) synchronizes */
+int spk_closure_grads(spk_closure c, const void *u, const void *tau_true, double *loss);
+int spk_closure_get_grads(spk_closure c, float *host_grads); /* synchronizes */
+/* spk_closure_grads then one Adam step (beta 0.9, 0.999, eps 1e-8) at rate lr */
+int spk_closure_train_step(spk_closure c, const void *u, const void *tau_true, double lr, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

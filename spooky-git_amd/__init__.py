@@ -120,6 +120,16 @@ SIGNATURES = {
     "spk_ensemble_set_graph": [_VP, _I],
     "spk_ensemble_set_lanes": [_VP, _I],
     "spk_ensemble_tune": [_VP, _I, _VP, _VP],
+    "spk_closure_num_params": [],
+    "spk_closure_create": [_VP, _I, _VP],
+    "spk_closure_destroy": [_VP],
+    "spk_closure_set_params": [_VP, _VP],
+    "spk_closure_get_params": [_VP, _VP],
+    "spk_closure_set_scales": [_VP, _D, _VP],
+    "spk_closure_forward": [_VP, _VP, _VP],
+    "spk_closure_grads": [_VP, _VP, _VP, _VP],
+    "spk_closure_get_grads": [_VP, _VP],
+    "spk_closure_train_step": [_VP, _VP, _VP, _D, _VP],
 }
 
 
@@ -138,6 +148,7 @@ def lib():
             fn.restype = ctypes.c_int
         L.spk_last_error.restype = ctypes.c_char_p
         L.spk_params_size.restype = ctypes.c_size_t
+        L.spk_closure_num_params.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
@@ -491,6 +502,74 @@ class Ensemble:
             lib().spk_ensemble_destroy(self.h)
             self.h = None
             self.ctx.close()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Closure:
+    """Learned subgrid-stress closure (spk_closure): a 4-layer 3D conv net on a
+    periodic n^3 grid, BF16 on the matrix cores.  Velocities and stresses are
+    device buffers in the padded real FP64 layout [3 or 6][n][n][n + 2]."""
+
+    CIN = (16, 32, 32, 32)
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, n
+        h = ctypes.c_void_p()
+        check(lib().spk_closure_create(ctx.h, n, ctypes.byref(h)))
+        self.h = h
+        self.nparams = int(lib().spk_closure_num_params())
+
+    @classmethod
+    def layer_slices(cls):
+        """(W, b) slices of each layer in the flat parameter vector."""
+        out, o = [], 0
+        for cin in cls.CIN:
+            nw = 27 * 32 * cin
+            out.append((slice(o, o + nw), slice(o + nw, o + nw + 32)))
+            o += nw + 32
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        check(lib().spk_closure_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        check(lib().spk_closure_get_params(self.h, p.ctypes.data_as(ctypes.c_void_p)))
+        return p
+
+    def get_grads(self):
+        g = np.empty(self.nparams, np.float32)
+        check(lib().spk_closure_get_grads(self.h, g.ctypes.data_as(ctypes.c_void_p)))
+        return g
+
+    def set_scales(self, in_scale, out_scale):
+        o = (ctypes.c_double * 6)(*[float(x) for x in out_scale])
+        check(lib().spk_closure_set_scales(self.h, float(in_scale), o))
+
+    def forward(self, d_u, d_tau):
+        check(lib().spk_closure_forward(self.h, d_u, d_tau))
+
+    def grads(self, d_u, d_tau):
+        loss = ctypes.c_double()
+        check(lib().spk_closure_grads(self.h, d_u, d_tau, ctypes.byref(loss)))
+        return loss.value
+
+    def train_step(self, d_u, d_tau, lr, want_loss=True):
+        loss = ctypes.c_double()
+        check(lib().spk_closure_train_step(self.h, d_u, d_tau, float(lr), ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def destroy(self):
+        if self.h:
+            lib().spk_closure_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

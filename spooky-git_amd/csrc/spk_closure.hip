@@ -1,0 +1,628 @@
+// spk_closure.hip -- learned subgrid-stress closure: a 4-layer 3D convolution
+// network on the periodic n^3 grid of a filtered velocity field, BF16 weights
+// and activations on the gfx950 matrix cores (v_mfma_f32_32x32x16_bf16) with
+// FP32 accumulation; forward, backward (input and weight gradients) and Adam
+// written here.
+//
+//   layer 1: 3 (padded to 16) -> 32 channels, 3^3 taps, GELU
+//   layers 2, 3: 32 -> 32, GELU
+//   layer 4: 32 -> 6 (padded to 32), linear: the normalized stress
+//            tau_xx, tau_xy, tau_xz, tau_yy, tau_yz, tau_zz
+// Activations are channels-last [point][channel] BF16, point = (x n + y) n + z.
+//
+// Convolution as a GEMM per 4x4x32 brick of points: D[co][point] +=
+// W[tap][co][ci-chunk] x X[point + tap][ci-chunk], 27 taps x (CIN / 16)
+// K-chunks of 16 channels.  A block (4 waves, one x each, four y columns of 32
+// z points per wave) keeps all 27 taps' weights in LDS for its lifetime and
+// stages each brick's input with a one-point halo; blocks loop over bricks.
+// 16-byte LDS reads are conflict-free through an XOR swizzle of the 8-channel
+// chunks by point index.  The weight gradient contracts over points instead:
+// dW[tap][co][ci] = sum_p dZ[p][co] X[p + tap][ci], with both operands staged
+// channel-major so that a lane's 8 K values are 8 consecutive z points; the
+// +-1 z shift of a tap is made with 16-bit funnel shifts of aligned loads.
+#include "spk_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace spk {
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int HX = 6, HY = 6, HZ = 34;  // brick 4 x 4 x 32 with a one-point halo
+constexpr int CH = 32;                   // hidden channels
+constexpr int NTAP = 27;
+constexpr int NBLK = 256;                // persistent blocks (one per CU)
+
+__host__ __device__ constexpr int swz_shift(int cin) { return cin == 32 ? 2 : 3; }
+__device__ __forceinline__ int swz(int q, int p, int cin) { return q ^ ((p >> swz_shift(cin)) & (cin / 8 - 1)); }
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+constexpr size_t conv_lds(int cin) { return (size_t)(NTAP * CH * cin + HX * HY * HZ * cin) * sizeof(bf16); }
+
+// MODE 0: hidden layer, Y = GELU(conv + b); 1: also Z = conv + b (training);
+// 2: output layer, out[c][p] = conv + b for c < 6 (FP32, channel-major);
+// 3: input gradient, Y = conv(dZ, flipped transposed W) * GELU'(Zp)
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                              const float *__restrict__ bias, bf16 *__restrict__ Y,
+                                              bf16 *__restrict__ Z, const bf16 *__restrict__ Zp,
+                                              float *__restrict__ out, int n) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    bf16 *Ws = reinterpret_cast<bf16 *>(smem);  // [27][32][CIN], chunk-swizzled by co
+    bf16 *Xs = Ws + NTAP * CH * CIN;            // [HX*HY*HZ][CIN], chunk-swizzled by halo point
+    constexpr int NQ = CIN / 8;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 31, half = lane >> 5;
+
+    for (int i = tid; i < NTAP * CH * NQ; i += 256) {
+        const int q = i % NQ, row = i / NQ;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + (size_t)row * CIN + 8 * q);
+        *reinterpret_cast<bf16x8 *>(Ws + row * CIN + 8 * swz(q, row & 31, CIN)) = v;
+    }
+    const int nbz = n / 32, nby = n / 4;
+    const int nb = (n / 4) * nby * nbz;
+    const size_t P = (size_t)n * n * n;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
+        __syncthreads();
+        for (int i = tid; i < HX * HY * HZ * NQ; i += 256) {
+            const int q = i % NQ, hp = i / NQ;
+            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
+            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
+            const size_t g = (((size_t)gx * n + gy) * n + gz) * CIN + 8 * q;
+            *reinterpret_cast<bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN)) =
+                *reinterpret_cast<const bf16x8 *>(X + g);
+        }
+        __syncthreads();
+        f32x16 acc[4];
+#pragma unroll
+        for (int ys = 0; ys < 4; ys++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[ys][r] = 0.0f;
+        for (int t = 0; t < NTAP; t++) {
+            const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+#pragma unroll
+            for (int kc = 0; kc < CIN / 16; kc++) {
+                const int q = 2 * kc + half;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Ws + (t * CH + li) * CIN + 8 * swz(q, li, CIN));
+#pragma unroll
+                for (int ys = 0; ys < 4; ys++) {
+                    const int hp = ((w + 1 + dx) * HY + (ys + 1 + dy)) * HZ + li + 1 + dz;
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN));
+                    acc[ys] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[ys], 0, 0, 0);
+                }
+            }
+        }
+        // D[co][point]: lane holds point li, co = 8 (r / 4) + 4 half + r % 4
+#pragma unroll
+        for (int ys = 0; ys < 4; ys++) {
+            const size_t p = ((size_t)(x0 + w) * n + (y0 + ys)) * n + z0 + li;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int c0 = 8 * g + 4 * half;
+                if (MODE == 2) {
+                    if (c0 < 6) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (c0 + j < 6) out[(size_t)(c0 + j) * P + p] = acc[ys][4 * g + j] + bias[c0 + j];
+                    }
+                } else if (MODE == 3) {
+                    const bf16x4 zp = *reinterpret_cast<const bf16x4 *>(Zp + p * CH + c0);
+                    bf16x4 y;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) y[j] = (bf16)(acc[ys][4 * g + j] * gelu_d((float)zp[j]));
+                    *reinterpret_cast<bf16x4 *>(Y + p * CH + c0) = y;
+                } else {
+                    bf16x4 y, zz;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float zf = acc[ys][4 * g + j] + bias[c0 + j];
+                        zz[j] = (bf16)zf;
+                        y[j] = (bf16)gelu(zf);
+                    }
+                    *reinterpret_cast<bf16x4 *>(Y + p * CH + c0) = y;
+                    if (MODE == 1) *reinterpret_cast<bf16x4 *>(Z + p * CH + c0) = zz;
+                }
+            }
+        }
+    }
+}
+
+// weight gradient, per-block partial sums part[block][27][32 co][32 ci]
+constexpr int DZS = 512 + 8;                 // dZ^T row (one co): 512 brick points + pad (65 16-byte slots)
+constexpr int XPL = HX * HY * 48 + 8;        // X^T plane (one ci): z rows of 48 + pad (217 slots)
+constexpr size_t DW_LDS = (size_t)(CH * DZS + CH * XPL) * sizeof(bf16);
+
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
+                                                 float *__restrict__ part, int n) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    bf16 *Ds = reinterpret_cast<bf16 *>(smem);  // [32 co][DZS]: point (bx, by, z) at (bx 4 + by) 32 + z
+    bf16 *Xs = Ds + CH * DZS;                   // [32 ci][XPL]: (hx, hy, z0 - 8 + e) at (hx 6 + hy) 48 + e
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 31, half = lane >> 5;
+    for (int i = tid; i < CH * XPL; i += 256) Xs[i] = (bf16)0.0f;  // padding and channels >= CIN stay 0
+    f32x16 acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[k][r] = 0.0f;
+    const int nbz = n / 32, nby = n / 4;
+    const int nb = (n / 4) * nby * nbz;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
+        __syncthreads();
+        for (int i = tid; i < 512 * 4; i += 256) {
+            const int q = i & 3, pt = i >> 2;
+            const size_t p = ((size_t)(x0 + (pt >> 7)) * n + y0 + ((pt >> 5) & 3)) * n + z0 + (pt & 31);
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
+#pragma unroll
+            for (int j = 0; j < 8; j++) Ds[(8 * q + j) * DZS + pt] = v[j];
+        }
+        constexpr int NQ = CIN / 8;
+        for (int i = tid; i < HX * HY * HZ * NQ; i += 256) {
+            const int q = i % NQ, hp = i / NQ;
+            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
+            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
+            const bf16x8 v =
+                *reinterpret_cast<const bf16x8 *>(X + (((size_t)gx * n + gy) * n + gz) * CIN + 8 * q);
+            const int e = (hx * HY + hy) * 48 + 7 + hz;
+#pragma unroll
+            for (int j = 0; j < 8; j++) Xs[(8 * q + j) * XPL + e] = v[j];
+        }
+        __syncthreads();
+        for (int s = 0; s < 32; s++) {
+            const int run = 2 * s + half, col = run >> 2, zr = (run & 3) * 8;
+            const int bxl = col >> 2, byl = col & 3;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Ds + li * DZS + col * 32 + zr);
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                const int t = w + 4 * k;
+                if (t < NTAP) {
+                    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+                    const int base = li * XPL + ((bxl + 1 + dx) * HY + (byl + 1 + dy)) * 48 + 8 + zr;
+                    const u32x4 d = *reinterpret_cast<const u32x4 *>(Xs + base);
+                    u32x4 sv = d;
+                    if (dz > 0) {
+                        const unsigned ex = *reinterpret_cast<const unsigned *>(Xs + base + 8);
+                        sv[0] = __builtin_amdgcn_alignbit(d[1], d[0], 16);
+                        sv[1] = __builtin_amdgcn_alignbit(d[2], d[1], 16);
+                        sv[2] = __builtin_amdgcn_alignbit(d[3], d[2], 16);
+                        sv[3] = __builtin_amdgcn_alignbit(ex, d[3], 16);
+                    } else if (dz < 0) {
+                        const unsigned ex = *reinterpret_cast<const unsigned *>(Xs + base - 2);
+                        sv[0] = __builtin_amdgcn_alignbit(d[0], ex, 16);
+                        sv[1] = __builtin_amdgcn_alignbit(d[1], d[0], 16);
+                        sv[2] = __builtin_amdgcn_alignbit(d[2], d[1], 16);
+                        sv[3] = __builtin_amdgcn_alignbit(d[3], d[2], 16);
+                    }
+                    bf16x8 bv;
+                    __builtin_memcpy(&bv, &sv, 16);
+                    acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[k], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[co][ci]: lane holds ci = li, co = 8 (r / 4) + 4 half + r % 4
+    float *pb = part + (size_t)blockIdx.x * NTAP * CH * CH;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const int t = w + 4 * k;
+        if (t < NTAP) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int co = 8 * (r / 4) + 4 * half + (r & 3);
+                pb[((size_t)t * CH + co) * CH + li] = acc[k][r];
+            }
+        }
+    }
+}
+
+// bias gradient partials: part[block][32] = sum over the block's points of dZ
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ dZ, size_t P, float *__restrict__ part) {
+    __shared__ float sm[CH];
+    if (threadIdx.x < CH) sm[threadIdx.x] = 0.0f;
+    __syncthreads();
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; c++) acc[c] = 0.0f;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc[8 * q + j] += (float)v[j];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; c++) atomicAdd(&sm[c], acc[c]);
+    __syncthreads();
+    if (threadIdx.x < CH) part[(size_t)blockIdx.x * CH + threadIdx.x] = sm[threadIdx.x];
+}
+
+// grad[k] = sum_b part[b][k] for k < count
+__global__ void k_sum_parts(const float *__restrict__ part, int nparts, int count, float *__restrict__ grad) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    float s = 0.0f;
+    for (int b = 0; b < nparts; b++) s += part[(size_t)b * count + k];
+    grad[k] = s;
+}
+
+// padded real FP64 u [3][n][n][n + 2] -> BF16 [P][16] (channels 3..15 zero), times in_scale
+__global__ void k_pack_input(const double *__restrict__ u, bf16 *__restrict__ xin, int n, float in_scale) {
+    const size_t P = (size_t)n * n * n;
+    const size_t fs = (size_t)n * n * (n + 2);
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = p / n, z = p - r * n;
+        const size_t o = r * (n + 2) + z;
+        bf16x8 v0, v1;
+        for (int j = 0; j < 8; j++) v0[j] = v1[j] = (bf16)0.0f;
+        v0[0] = (bf16)((float)u[o] * in_scale);
+        v0[1] = (bf16)((float)u[fs + o] * in_scale);
+        v0[2] = (bf16)((float)u[2 * fs + o] * in_scale);
+        *reinterpret_cast<bf16x8 *>(xin + p * 16) = v0;
+        *reinterpret_cast<bf16x8 *>(xin + p * 16 + 8) = v1;
+    }
+}
+
+// normalized FP32 [6][P] <-> padded real FP64 [6][n][n][n + 2] with per-component scales
+__global__ void k_unpack_output(const float *__restrict__ out, double *__restrict__ tau, int n, double s0, double s1,
+                                double s2, double s3, double s4, double s5) {
+    const size_t P = (size_t)n * n * n;
+    const size_t fs = (size_t)n * n * (n + 2);
+    const double sc[6] = {s0, s1, s2, s3, s4, s5};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 6 * fs; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / fs);
+        const size_t e = i - (size_t)c * fs;
+        const size_t r = e / (n + 2), z = e - r * (n + 2);
+        tau[i] = z < (size_t)n ? (double)out[(size_t)c * P + r * n + z] * sc[c] : 0.0;
+    }
+}
+__global__ void k_prep_target(const double *__restrict__ tau, float *__restrict__ tgt, int n, double s0, double s1,
+                              double s2, double s3, double s4, double s5) {
+    const size_t P = (size_t)n * n * n;
+    const size_t fs = (size_t)n * n * (n + 2);
+    const double sc[6] = {s0, s1, s2, s3, s4, s5};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 6 * P; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / P);
+        const size_t p = i - (size_t)c * P;
+        const size_t r = p / n, z = p - r * n;
+        tgt[i] = (float)(tau[(size_t)c * fs + r * (n + 2) + z] / sc[c]);
+    }
+}
+
+// loss = mean over 6 P of (pred - tgt)^2; dZ4[p][c] = 2 (pred - tgt) / (6 P) (c < 6, else 0)
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ pred, const float *__restrict__ tgt,
+                                              bf16 *__restrict__ dz, size_t P, float *__restrict__ part) {
+    __shared__ float sm[256];
+    float acc = 0.0f;
+    const float g = 2.0f / (6.0f * (float)P);
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        bf16x8 v0, v1, v2, v3;
+        for (int j = 0; j < 8; j++) v0[j] = v1[j] = v2[j] = v3[j] = (bf16)0.0f;
+        for (int c = 0; c < 6; c++) {
+            const float d = pred[(size_t)c * P + p] - tgt[(size_t)c * P + p];
+            acc += d * d;
+            v0[c] = (bf16)(g * d);
+        }
+        bf16 *o = dz + p * CH;
+        *reinterpret_cast<bf16x8 *>(o) = v0;
+        *reinterpret_cast<bf16x8 *>(o + 8) = v1;
+        *reinterpret_cast<bf16x8 *>(o + 16) = v2;
+        *reinterpret_cast<bf16x8 *>(o + 24) = v3;
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// Adam (Kingma & Ba) on the FP32 master parameters
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       int count, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    w[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// BF16 weights of one layer from its FP32 master weights W[27][32][cin]; and the
+// flipped transpose WT[26 - t][ci][co] = W[t][co][ci] (cin = 32) for the input gradient
+__global__ void k_pack_weights(const float *__restrict__ W, bf16 *__restrict__ Wb, bf16 *__restrict__ WT, int cin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NTAP * CH * cin) return;
+    const int ci = k % cin, co = (k / cin) % CH, t = k / (cin * CH);
+    const bf16 v = (bf16)W[k];
+    Wb[k] = v;
+    if (WT) WT[((NTAP - 1 - t) * CH + ci) * CH + co] = v;
+}
+
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+// parameter layout: for layer l = 1..4: W_l[27][32][cin_l] then b_l[32]; cin = 16, 32, 32, 32
+static const int CIN_L[4] = {16, 32, 32, 32};
+static size_t w_count(int l) { return (size_t)NTAP * CH * CIN_L[l]; }
+static size_t param_offset(int l) {
+    size_t o = 0;
+    for (int i = 0; i < l; i++) o += w_count(i) + CH;
+    return o;
+}
+static size_t param_total() { return param_offset(4); }
+
+struct spk_closure_s {
+    spk_ctx ctx = nullptr;
+    int n = 0;
+    size_t P = 0;
+    double in_scale = 1.0, out_scale[6] = {1, 1, 1, 1, 1, 1};
+    float *prm = nullptr, *grd = nullptr, *am = nullptr, *av = nullptr;  // FP32 master, gradient, Adam moments
+    bf16 *Wb[4] = {}, *WT[4] = {};
+    bf16 *xin = nullptr, *Z[3] = {}, *A[3] = {}, *dzA = nullptr, *dzB = nullptr;
+    float *out = nullptr, *tgt = nullptr, *part = nullptr, *bpart = nullptr, *lpart = nullptr;
+    int adam_t = 0;
+};
+
+namespace {
+
+int closure_alloc(spk_closure c) {
+    auto al = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            set_error("spk_closure: hipMalloc(%zu) failed", bytes);
+            return SPK_ERR_ALLOC;
+        }
+        return SPK_OK;
+    };
+    const size_t NP = param_total();
+    SPK_TRY(al((void **)&c->prm, NP * 4));
+    SPK_TRY(al((void **)&c->grd, NP * 4));
+    SPK_TRY(al((void **)&c->am, NP * 4));
+    SPK_TRY(al((void **)&c->av, NP * 4));
+    for (int l = 0; l < 4; l++) {
+        SPK_TRY(al((void **)&c->Wb[l], w_count(l) * 2));
+        if (l > 0) SPK_TRY(al((void **)&c->WT[l], w_count(l) * 2));
+    }
+    SPK_TRY(al((void **)&c->xin, c->P * 16 * 2));
+    for (int l = 0; l < 3; l++) {
+        SPK_TRY(al((void **)&c->Z[l], c->P * CH * 2));
+        SPK_TRY(al((void **)&c->A[l], c->P * CH * 2));
+    }
+    SPK_TRY(al((void **)&c->dzA, c->P * CH * 2));
+    SPK_TRY(al((void **)&c->dzB, c->P * CH * 2));
+    SPK_TRY(al((void **)&c->out, c->P * 6 * 4));
+    SPK_TRY(al((void **)&c->tgt, c->P * 6 * 4));
+    SPK_TRY(al((void **)&c->part, (size_t)NBLK * NTAP * CH * CH * 4));
+    SPK_TRY(al((void **)&c->bpart, (size_t)NBLK * CH * 4));
+    SPK_TRY(al((void **)&c->lpart, (size_t)NBLK * 4));
+    SPK_HIP(hipMemsetAsync(c->prm, 0, NP * 4, c->ctx->stream));
+    SPK_HIP(hipMemsetAsync(c->am, 0, NP * 4, c->ctx->stream));
+    SPK_HIP(hipMemsetAsync(c->av, 0, NP * 4, c->ctx->stream));
+    return SPK_OK;
+}
+
+void closure_free(spk_closure c) {
+    void *b[] = {c->prm, c->grd, c->am, c->av, c->xin, c->dzA, c->dzB, c->out, c->tgt, c->part, c->bpart, c->lpart};
+    for (void *p : b)
+        if (p) (void)hipFree(p);
+    for (int l = 0; l < 4; l++) {
+        if (c->Wb[l]) (void)hipFree(c->Wb[l]);
+        if (c->WT[l]) (void)hipFree(c->WT[l]);
+    }
+    for (int l = 0; l < 3; l++) {
+        if (c->Z[l]) (void)hipFree(c->Z[l]);
+        if (c->A[l]) (void)hipFree(c->A[l]);
+    }
+}
+
+int set_lds_attributes() {
+    static bool done = false;
+    if (done) return SPK_OK;
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<16, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(16)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<16, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(16)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv_dw<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)DW_LDS));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv_dw<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)DW_LDS));
+    done = true;
+    return SPK_OK;
+}
+
+int pack_weights(spk_closure c) {
+    for (int l = 0; l < 4; l++) {
+        const int cnt = (int)w_count(l);
+        k_pack_weights<<<(cnt + 255) / 256, 256, 0, c->ctx->stream>>>(c->prm + param_offset(l), c->Wb[l], c->WT[l],
+                                                                     CIN_L[l]);
+    }
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+unsigned conv_grid(spk_closure c) {
+    const int nb = (c->n / 4) * (c->n / 4) * (c->n / 32);
+    return (unsigned)(nb < NBLK ? nb : NBLK);
+}
+
+int forward(spk_closure c, const void *u, bool train) {
+    hipStream_t st = c->ctx->stream;
+    const unsigned gp = (unsigned)((c->P + 255) / 256 < 4096 ? (c->P + 255) / 256 : 4096);
+    k_pack_input<<<gp, 256, 0, st>>>((const double *)u, c->xin, c->n, (float)c->in_scale);
+    const unsigned g = conv_grid(c);
+    const float *b[4];
+    for (int l = 0; l < 4; l++) b[l] = c->prm + param_offset(l) + w_count(l);
+    if (train) {
+        k_conv<16, 1><<<g, 256, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], c->Z[0], nullptr, nullptr, c->n);
+        k_conv<32, 1><<<g, 256, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], c->Z[1], nullptr, nullptr, c->n);
+        k_conv<32, 1><<<g, 256, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], c->Z[2], nullptr, nullptr, c->n);
+    } else {
+        k_conv<16, 0><<<g, 256, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], nullptr, nullptr, nullptr, c->n);
+        k_conv<32, 0><<<g, 256, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], nullptr, nullptr, nullptr, c->n);
+        k_conv<32, 0><<<g, 256, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], nullptr, nullptr, nullptr, c->n);
+    }
+    k_conv<32, 2><<<g, 256, conv_lds(32), st>>>(c->A[2], c->Wb[3], b[3], nullptr, nullptr, nullptr, c->out, c->n);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+// weight and bias gradients of layer l from its input X and its dZ, into grd
+int layer_grads(spk_closure c, int l, const bf16 *X, const bf16 *dz) {
+    hipStream_t st = c->ctx->stream;
+    const unsigned g = conv_grid(c);
+    if (CIN_L[l] == 16) k_conv_dw<16><<<g, 256, DW_LDS, st>>>(X, dz, c->part, c->n);
+    else k_conv_dw<32><<<g, 256, DW_LDS, st>>>(X, dz, c->part, c->n);
+    const int cnt = NTAP * CH * CH;
+    // the [27][32][32] sums land in grd for cin = 32; for cin = 16 take ci < 16
+    float *gw = c->grd + param_offset(l);
+    if (CIN_L[l] == 32) {
+        k_sum_parts<<<(cnt + 255) / 256, 256, 0, st>>>(c->part, (int)g, cnt, gw);
+    } else {
+        k_sum_parts<<<(cnt + 255) / 256, 256, 0, st>>>(c->part, (int)g, cnt, c->out);  // out is free here
+        SPK_HIP(hipMemcpy2DAsync(gw, 16 * 4, c->out, 32 * 4, 16 * 4, NTAP * CH, hipMemcpyDeviceToDevice, st));
+    }
+    k_bias_grad<<<NBLK, 256, 0, st>>>(dz, c->P, c->bpart);
+    k_sum_parts<<<1, CH, 0, st>>>(c->bpart, NBLK, CH, gw + w_count(l));
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" size_t spk_closure_num_params(void) { return param_total(); }
+
+extern "C" int spk_closure_create(spk_ctx ctx, int n, spk_closure *out) {
+    SPK_REQUIRE(ctx && out, SPK_ERR_ARG, "spk_closure_create: null argument");
+    SPK_REQUIRE(n >= 32 && n % 32 == 0 && n <= 1024, SPK_ERR_ARG,
+                "spk_closure_create: n = %d (a multiple of 32, 32..1024)", n);
+    SPK_TRY(set_lds_attributes());
+    spk_closure c = new spk_closure_s();
+    c->ctx = ctx;
+    c->n = n;
+    c->P = (size_t)n * n * n;
+    const int rc = closure_alloc(c);
+    if (rc != SPK_OK) {
+        closure_free(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return pack_weights(c);
+}
+
+extern "C" int spk_closure_destroy(spk_closure c) {
+    if (!c) return SPK_OK;
+    (void)hipStreamSynchronize(c->ctx->stream);
+    closure_free(c);
+    delete c;
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_set_params(spk_closure c, const float *host) {
+    SPK_REQUIRE(c && host, SPK_ERR_ARG, "spk_closure_set_params: null argument");
+    SPK_HIP(hipMemcpyAsync(c->prm, host, param_total() * 4, hipMemcpyHostToDevice, c->ctx->stream));
+    SPK_HIP(hipMemsetAsync(c->am, 0, param_total() * 4, c->ctx->stream));
+    SPK_HIP(hipMemsetAsync(c->av, 0, param_total() * 4, c->ctx->stream));
+    c->adam_t = 0;
+    SPK_TRY(pack_weights(c));
+    SPK_HIP(hipStreamSynchronize(c->ctx->stream));
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_get_params(spk_closure c, float *host) {
+    SPK_REQUIRE(c && host, SPK_ERR_ARG, "spk_closure_get_params: null argument");
+    SPK_HIP(hipMemcpyAsync(host, c->prm, param_total() * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(c->ctx->stream));
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_get_grads(spk_closure c, float *host) {
+    SPK_REQUIRE(c && host, SPK_ERR_ARG, "spk_closure_get_grads: null argument");
+    SPK_HIP(hipMemcpyAsync(host, c->grd, param_total() * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(c->ctx->stream));
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_set_scales(spk_closure c, double in_scale, const double *out_scale) {
+    SPK_REQUIRE(c && out_scale && in_scale > 0, SPK_ERR_ARG, "spk_closure_set_scales: bad arguments");
+    c->in_scale = in_scale;
+    for (int i = 0; i < 6; i++) {
+        SPK_REQUIRE(out_scale[i] > 0, SPK_ERR_ARG, "spk_closure_set_scales: out_scale[%d] <= 0", i);
+        c->out_scale[i] = out_scale[i];
+    }
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_forward(spk_closure c, const void *u, void *tau) {
+    SPK_REQUIRE(c && u && tau, SPK_ERR_ARG, "spk_closure_forward: null argument");
+    SPK_TRY(forward(c, u, false));
+    const double *s = c->out_scale;
+    k_unpack_output<<<4096, 256, 0, c->ctx->stream>>>(c->out, (double *)tau, c->n, s[0], s[1], s[2], s[3], s[4], s[5]);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_grads(spk_closure c, const void *u, const void *tau_true, double *loss) {
+    SPK_REQUIRE(c && u && tau_true, SPK_ERR_ARG, "spk_closure_grads: null argument");
+    hipStream_t st = c->ctx->stream;
+    SPK_TRY(forward(c, u, true));
+    const double *s = c->out_scale;
+    k_prep_target<<<4096, 256, 0, st>>>((const double *)tau_true, c->tgt, c->n, s[0], s[1], s[2], s[3], s[4], s[5]);
+    k_loss<<<NBLK, 256, 0, st>>>(c->out, c->tgt, c->dzA, c->P, c->lpart);
+    SPK_LAUNCH_CHECK();
+    const unsigned g = conv_grid(c);
+    // layer 4, then back through layers 3, 2, 1
+    SPK_TRY(layer_grads(c, 3, c->A[2], c->dzA));
+    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzA, c->WT[3], nullptr, c->dzB, nullptr, c->Z[2], nullptr, c->n);
+    SPK_TRY(layer_grads(c, 2, c->A[1], c->dzB));
+    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzB, c->WT[2], nullptr, c->dzA, nullptr, c->Z[1], nullptr, c->n);
+    SPK_TRY(layer_grads(c, 1, c->A[0], c->dzA));
+    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzA, c->WT[1], nullptr, c->dzB, nullptr, c->Z[0], nullptr, c->n);
+    SPK_TRY(layer_grads(c, 0, c->xin, c->dzB));
+    SPK_LAUNCH_CHECK();
+    if (loss) {
+        std::vector<float> h(NBLK);
+        SPK_HIP(hipMemcpyAsync(h.data(), c->lpart, NBLK * 4, hipMemcpyDeviceToHost, st));
+        SPK_HIP(hipStreamSynchronize(st));
+        double sum = 0.0;
+        for (float v : h) sum += v;
+        *loss = sum / (6.0 * (double)c->P);
+    }
+    return SPK_OK;
+}
+
+extern "C" int spk_closure_train_step(spk_closure c, const void *u, const void *tau_true, double lr, double *loss) {
+    SPK_TRY(spk_closure_grads(c, u, tau_true, loss));
+    const int NP = (int)param_total();
+    c->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)c->adam_t), c2 = 1.0f - powf(b2, (float)c->adam_t);
+    k_adam<<<(NP + 255) / 256, 256, 0, c->ctx->stream>>>(c->prm, c->grd, c->am, c->av, NP, (float)lr, b1, b2, 1e-8f,
+                                                         c1, c2);
+    SPK_LAUNCH_CHECK();
+    return pack_weights(c);
+}

@@ -1,0 +1,240 @@
+"""Learned subgrid-stress closure (spk_closure) on the GPU against numpy.
+
+The numpy restatement rounds to BF16 where the GPU stores BF16 (the input, the
+weights, every hidden activation, the stored pre-activations and the
+back-propagated gradients) and computes everything else in float64; the GPU
+accumulates in FP32 on the matrix cores.  Tolerances are relative L2 over a
+whole tensor.  Known answers that do not go through the restatement: biases
+alone give a constant stress; a one-tap network is a shift of the input by
+(dx, dy, dz) composed with GELU; the output-bias gradient is the mean error."""
+import numpy as np
+import pytest
+from scipy.special import erf
+
+from spkmod import spk
+
+pytestmark = pytest.mark.gpu
+
+N = 32
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = spk.Ctx(0)
+    yield c
+    c.close()
+
+
+def bf(x):
+    """Round to the nearest BF16 (ties to even), returned as float64."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def gelu(z):
+    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+
+
+def gelu_d(z):
+    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+
+
+def taps():
+    for t in range(27):
+        yield t, (t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1)
+
+
+def conv(X, W):
+    """Y[p][co] = sum_t sum_ci X[p + d_t][ci] W[t][co][ci] on the periodic grid."""
+    Y = np.zeros(X.shape[:3] + (W.shape[1],))
+    for t, d in taps():
+        Y += np.roll(X, shift=(-d[0], -d[1], -d[2]), axis=(0, 1, 2)) @ W[t].T
+    return Y
+
+
+def conv_t(D, W):
+    """dX[q][ci] = sum_t sum_co D[q - d_t][co] W[t][co][ci]."""
+    X = np.zeros(D.shape[:3] + (W.shape[2],))
+    for t, d in taps():
+        X += np.roll(D, shift=d, axis=(0, 1, 2)) @ W[t]
+    return X
+
+
+def conv_dw(D, X):
+    """dW[t][co][ci] = sum_p D[p][co] X[p + d_t][ci]."""
+    P = D.shape[0] * D.shape[1] * D.shape[2]
+    out = np.zeros((27, D.shape[3], X.shape[3]))
+    for t, d in taps():
+        Xs = np.roll(X, shift=(-d[0], -d[1], -d[2]), axis=(0, 1, 2))
+        out[t] = D.reshape(P, -1).T @ Xs.reshape(P, -1)
+    return out
+
+
+def unpack(params):
+    Ws, bs = [], []
+    for (ws, bsl), cin in zip(spk.Closure.layer_slices(), spk.Closure.CIN):
+        Ws.append(params[ws].astype(np.float64).reshape(27, 32, cin))
+        bs.append(params[bsl].astype(np.float64))
+    return Ws, bs
+
+
+def reference(params, u, in_scale, tgt=None):
+    """numpy forward (and backward when tgt is given); u: [3][n][n][n]."""
+    Ws, bs = unpack(params)
+    Wb = [bf(w) for w in Ws]
+    x0 = np.zeros((N, N, N, 16))
+    x0[..., :3] = bf(np.moveaxis(u.astype(np.float32) * np.float32(in_scale), 0, -1))
+    acts, zs = [x0], []
+    for l in range(3):
+        z = conv(acts[-1], Wb[l]) + bs[l]
+        zs.append(bf(z))
+        acts.append(bf(gelu(z)))
+    out = conv(acts[-1], Wb[3])[..., :6] + bs[3][:6]
+    if tgt is None:
+        return out
+    P = N ** 3
+    d = np.zeros((N, N, N, 32))
+    d[..., :6] = bf((2.0 / (6 * P)) * (out - tgt))
+    grads = [None] * 4
+    for l in (3, 2, 1, 0):
+        gw = conv_dw(d, acts[l])
+        grads[l] = (gw, d.reshape(P, 32).sum(0))
+        if l > 0:
+            d = bf(conv_t(d, Wb[l]) * gelu_d(zs[l - 1]))
+    loss = np.mean((out - tgt) ** 2)
+    return out, grads, loss
+
+
+def padded(fields):
+    f = np.asarray(fields)
+    out = np.zeros(f.shape[:-1] + (f.shape[-1] + 2,))
+    out[..., : f.shape[-1]] = f
+    return out
+
+
+def random_params(rng, scale=1.0):
+    p = np.zeros(sum(27 * 32 * c + 32 for c in spk.Closure.CIN), np.float32)
+    for (ws, bsl), cin in zip(spk.Closure.layer_slices(), spk.Closure.CIN):
+        w = rng.standard_normal((27, 32, cin)) * scale / np.sqrt(27 * 3 if cin == 16 else 27 * cin)
+        if cin == 16:
+            w[..., 3:] = 0.0
+        p[ws] = w.reshape(-1)
+        p[bsl] = 0.1 * rng.standard_normal(32)
+    ws, bsl = spk.Closure.layer_slices()[3]
+    w = p[ws].reshape(27, 32, 32)
+    w[:, 6:, :] = 0.0
+    p[ws] = w.reshape(-1)
+    p[bsl][6:] = 0.0
+    return p
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def device_fields(ctx, f):
+    buf = ctx.alloc(f.nbytes)
+    buf.upload(f)
+    return buf
+
+
+def test_forward_matches_numpy_and_known_answers(ctx):
+    rng = np.random.default_rng(1)
+    cl = spk.Closure(ctx, N)
+    assert cl.nparams == sum(27 * 32 * c + 32 for c in spk.Closure.CIN)
+    u = rng.standard_normal((3, N, N, N))
+    du = device_fields(ctx, padded(u))
+    dtau = ctx.alloc(6 * N * N * (N + 2) * 8)
+    scales = [1.0, 2.0, 0.5, 1.5, 3.0, 0.25]
+    cl.set_scales(0.7, scales)
+    # biases only: a constant stress
+    p = np.zeros(cl.nparams, np.float32)
+    p[spk.Closure.layer_slices()[3][1]][:6] = [1, 2, 3, 4, 5, 6]
+    cl.set_params(p)
+    cl.forward(du.ptr, dtau.ptr)
+    tau = dtau.download(np.float64, (6, N, N, N + 2))
+    for c in range(6):
+        assert np.all(tau[c, ..., :N] == (c + 1) * scales[c]) and np.all(tau[c, ..., N:] == 0)
+    # one tap per layer: tau_c = g(g(g(u_c(x + 1, y, z - 1)))) for c < 3 (layer 1 shifts by
+    # (+1, 0, -1): tap 9 * 2 + 3 * 1 + 0 = 21; layers 2-4 take the centre tap 13)
+    p = np.zeros(cl.nparams, np.float32)
+    sl = spk.Closure.layer_slices()
+    for l, (ws, _) in enumerate(sl):
+        cin = spk.Closure.CIN[l]
+        w = np.zeros((27, 32, cin), np.float32)
+        for c in range(3):
+            w[21 if l == 0 else 13, c, c] = 1.0
+        p[ws] = w.reshape(-1)
+    cl.set_params(p)
+    cl.set_scales(1.0, [1.0] * 6)
+    cl.forward(du.ptr, dtau.ptr)
+    tau = dtau.download(np.float64, (6, N, N, N + 2))[..., :N]
+    a = bf(np.roll(u.astype(np.float32), shift=(-1, 0, 1), axis=(1, 2, 3)))
+    for _ in range(3):
+        a = bf(gelu(a))
+    assert rel(tau[:3], a) < 1e-3 and np.all(tau[3:] == 0)
+    # random network against the numpy restatement
+    p = random_params(rng)
+    cl.set_params(p)
+    cl.set_scales(0.7, scales)
+    cl.forward(du.ptr, dtau.ptr)
+    tau = dtau.download(np.float64, (6, N, N, N + 2))[..., :N]
+    ref = np.moveaxis(reference(p, u, 0.7), -1, 0) * np.array(scales)[:, None, None, None]
+    assert rel(tau, ref) < 1e-2, rel(tau, ref)
+    cl.destroy()
+
+
+def test_gradients_match_numpy_backprop(ctx):
+    rng = np.random.default_rng(2)
+    cl = spk.Closure(ctx, N)
+    p = random_params(rng)
+    cl.set_params(p)
+    cl.set_scales(1.0, [1.0] * 6)
+    u = rng.standard_normal((3, N, N, N))
+    tgt = rng.standard_normal((6, N, N, N)) * 0.3
+    du = device_fields(ctx, padded(u))
+    dt = device_fields(ctx, padded(tgt))
+    loss = cl.grads(du.ptr, dt.ptr)
+    g = cl.get_grads()
+    out, grads, loss_ref = reference(p, u, 1.0, np.moveaxis(tgt, 0, -1))
+    assert loss == pytest.approx(loss_ref, rel=5e-3)
+    for l, (ws, bsl) in enumerate(spk.Closure.layer_slices()):
+        cin = spk.Closure.CIN[l]
+        gw_ref = grads[l][0][:, :, :cin] if cin == 16 else grads[l][0]
+        gw = g[ws].reshape(27, 32, cin)
+        if l == 3:  # output channels >= 6 have no gradient
+            assert np.all(gw[:, 6:, :] == 0)
+        assert rel(gw, gw_ref) < 2e-2, (l, rel(gw, gw_ref))
+        assert rel(g[bsl], grads[l][1]) < 2e-2, (l, rel(g[bsl], grads[l][1]))
+    # output-layer bias: dL/db_c = 2 mean_p (out_c - tgt_c) / 6, independent of the restatement
+    b4 = g[spk.Closure.layer_slices()[3][1]][:6]
+    direct = 2.0 * np.mean(out - np.moveaxis(tgt, 0, -1), axis=(0, 1, 2)) / 6.0
+    assert np.max(np.abs(b4 - direct)) < 1e-2 * np.max(np.abs(direct))
+    cl.destroy()
+
+
+def test_training_fits_a_teacher(ctx):
+    """A student network trained with Adam on a teacher network's output lowers
+    the loss below half in 150 steps (0.40 of the first on the MI355X)."""
+    rng = np.random.default_rng(3)
+    teacher = spk.Closure(ctx, N)
+    teacher.set_params(random_params(rng, 1.5))
+    teacher.set_scales(1.0, [1.0] * 6)
+    u = rng.standard_normal((3, N, N, N))
+    du = device_fields(ctx, padded(u))
+    dt = ctx.alloc(6 * N * N * (N + 2) * 8)
+    teacher.forward(du.ptr, dt.ptr)
+    ctx.sync()
+    student = spk.Closure(ctx, N)
+    student.set_params(random_params(rng, 1.0))
+    student.set_scales(1.0, [1.0] * 6)
+    first = student.train_step(du.ptr, dt.ptr, 3e-3)
+    for _ in range(148):
+        student.train_step(du.ptr, dt.ptr, 3e-3, want_loss=False)
+    last = student.train_step(du.ptr, dt.ptr, 3e-3)
+    print("teacher fit: loss %.4g -> %.4g" % (first, last))
+    assert np.isfinite(last) and last < 0.5 * first, (first, last)
+    teacher.destroy()
+    student.destroy()
