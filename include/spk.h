@@ -141,6 +141,12 @@ int spk_solver_set_fused(spk_solver s, int mode);
  * incompressible run and at most 64 stages at the cap.  Same results as the
  * host-planned cycle (on = 0, the default). */
 int spk_solver_set_rkl_device(spk_solver s, int on);
+/* Large-scale forcing (no counterpart in the reference): a fixed spectral
+ * field F on nmodes modes (index[m]: position in a field, 0 <= index < NTC;
+ * values[m][3][2]: the complex F of vx, vy, vz there); after every step of
+ * spk_main_loop the state gets U += dt F on those modes.  nmodes = 0 removes it.
+ * Incompressible runs only; synchronizes. */
+int spk_solver_set_forcing(spk_solver s, int nmodes, const long long *index, const double *values);
 /* The last device plan: its stage count, whether the step's count exceeded the
  * plan (overflow = 1: the cycle was cut short), and the stages launched per cycle. */
 int spk_solver_rkl_plan(spk_solver s, int *stages, int *overflow, int *smax);
@@ -256,6 +262,18 @@ int spk_closure_grads(spk_closure c, const void *u, const void *tau_true, double
 int spk_closure_get_grads(spk_closure c, float *host_grads); /* synchronizes */
 /* spk_closure_grads then one Adam step (beta 0.9, 0.999, eps 1e-8) at rate lr */
 int spk_closure_train_step(spk_closure c, const void *u, const void *tau_true, double lr, double *loss);
+/* The closure inline in a run on its grid (incompressible, same stream): after a
+ * step, U_i += dt P(-i mask k_j tau_ij(u)), tau from the network on the current
+ * real velocity, P the projection on k-perpendicular.  Overwrites the real buffer. */
+int spk_closure_apply(spk_closure c, spk_solver s);
+/* Training data from a fine run: with F the sharp cut to |k_i| <= kc,
+ * kc = (2/3)(n/2 - 1) (the dealiasing cut of the coarse grid), the coarse
+ * velocity ubar = F u and the subgrid stress tau_ij = F(u_i u_j) - F(ubar_i ubar_j)
+ * (products on the fine grid), on the coarse grid n^3.  U_hi: device spectral FP64
+ * velocity [3][N][N][N/2+1] (N a multiple of n); ubar_lo, tau_lo: device padded
+ * real FP64 [3][n][n][n + 2] and [6][n][n][n + 2].  Allocates fine-grid scratch
+ * (7 real fields of N^3) for the call; synchronizes. */
+int spk_sgs_stress(spk_ctx ctx, const void *U_hi, int N, void *ubar_lo, void *tau_lo, int n);
 
 #ifdef __cplusplus
 }

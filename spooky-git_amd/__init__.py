@@ -130,6 +130,9 @@ SIGNATURES = {
     "spk_closure_grads": [_VP, _VP, _VP, _VP],
     "spk_closure_get_grads": [_VP, _VP],
     "spk_closure_train_step": [_VP, _VP, _VP, _D, _VP],
+    "spk_closure_apply": [_VP, _VP],
+    "spk_sgs_stress": [_VP, _VP, _I, _VP, _VP, _I],
+    "spk_solver_set_forcing": [_VP, _I, _VP, _VP],
 }
 
 
@@ -433,6 +436,14 @@ class Solver:
         check(lib().spk_solver_rkl_plan(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def set_forcing(self, index, values):
+        """Fixed forcing F on the given modes (index: int64 positions in a field;
+        values: complex [nmodes, 3]); each step adds dt F."""
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        val = np.ascontiguousarray(np.asarray(values, dtype=np.complex128).reshape(len(idx), 3))
+        check(lib().spk_solver_set_forcing(self.h, len(idx), idx.ctypes.data_as(ctypes.c_void_p),
+                                           val.ctypes.data_as(ctypes.c_void_p)))
+
     def snapshot_real(self):
         """The real fields a snapshot holds into the real buffer (unsheared in a shearing box)."""
         check(lib().spk_snapshot_real(self.h))
@@ -566,6 +577,10 @@ class Closure:
         check(lib().spk_closure_train_step(self.h, d_u, d_tau, float(lr), ctypes.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 
+    def apply(self, sol):
+        """SGS term of the solver's state after its step (same stream as the solver)."""
+        check(lib().spk_closure_apply(self.h, sol.h))
+
     def destroy(self):
         if self.h:
             lib().spk_closure_destroy(self.h)
@@ -576,6 +591,11 @@ class Closure:
             self.destroy()
         except Exception:
             pass
+
+
+def sgs_stress(ctx, d_U_hi, N, d_ubar, d_tau, n):
+    """Filtered velocity and subgrid stress on n^3 from a fine N^3 spectral velocity (spk_sgs_stress)."""
+    check(lib().spk_sgs_stress(ctx.h, d_U_hi, N, d_ubar, d_tau, n))
 
 
 def padded_real(r):

@@ -21,6 +21,7 @@
 // channel-major so that a lane's 8 K values are 8 consecutive z points; the
 // +-1 z shift of a tap is made with 16-bit funnel shifts of aligned loads.
 #include "spk_internal.hpp"
+#include "spk_kernels.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -360,6 +361,97 @@ __global__ void k_pack_weights(const float *__restrict__ W, bf16 *__restrict__ W
     if (WT) WT[((NTAP - 1 - t) * CH + ci) * CH + co] = v;
 }
 
+// ---- filtered DNS data: truncation to a coarse grid, products, SGS divergence ----
+
+// coarse-grid index -> fine-grid index of the same wavenumber along one axis;
+// -1 where the coarse mode is cut (beyond |k| <= kc, or the coarse Nyquist)
+__device__ __forceinline__ long long fine_of(int i, int n, int N, int kc) {
+    const int k = i < n / 2 ? i : i - n;
+    if (i == n / 2 || (k < 0 ? -k : k) > kc) return -1;
+    return k >= 0 ? k : k + N;
+}
+
+// lo[f][i][j][k] = scale * hi[f][fine(i)][fine(j)][k] on the modes kept, 0 elsewhere
+// (complex, [NX][NY][NZ/2+1] layouts; hi at stride fs_hi, lo at stride fs_lo)
+__global__ void k_truncate(const cpx<double> *__restrict__ hi, int N, long long fs_hi, cpx<double> *__restrict__ lo,
+                           int n, long long fs_lo, int nf, int kc, double scale) {
+    const long long ntc = (long long)n * n * (n / 2 + 1);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ntc * nf;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int f = (int)(e / ntc);
+        const long long idx = e - (long long)f * ntc;
+        const int k = (int)(idx % (n / 2 + 1));
+        const long long r = idx / (n / 2 + 1);
+        const int j = (int)(r % n), i = (int)(r / n);
+        const long long fi = fine_of(i, n, N, kc), fj = fine_of(j, n, N, kc);
+        const bool keep = fi >= 0 && fj >= 0 && k < n / 2 && k <= kc;
+        cpx<double> v = {0.0, 0.0};
+        if (keep) {
+            const cpx<double> h = hi[(long long)f * fs_hi + (fi * N + fj) * (N / 2 + 1) + k];
+            v = cpx<double>{h.x * scale, h.y * scale};
+        }
+        lo[(long long)f * fs_lo + idx] = v;
+    }
+}
+
+// the same cut applied on the fine grid itself (modes beyond kc set to 0)
+__global__ void k_mask_fine(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ out, int N, long long fs,
+                            int nf, int kc) {
+    const long long ntc = (long long)N * N * (N / 2 + 1);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ntc * nf;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int f = (int)(e / ntc);
+        const long long idx = e - (long long)f * ntc;
+        const int k = (int)(idx % (N / 2 + 1));
+        const long long r = idx / (N / 2 + 1);
+        const int j = (int)(r % N), i = (int)(r / N);
+        const int kx = i < N / 2 ? i : i - N, ky = j < N / 2 ? j : j - N;
+        const bool keep = (kx < 0 ? -kx : kx) <= kc && (ky < 0 ? -ky : ky) <= kc && k <= kc;
+        out[(long long)f * fs + idx] = keep ? in[(long long)f * fs + idx] : cpx<double>{0.0, 0.0};
+    }
+}
+
+// padded real rows: out = a_i a_j - b_i b_j (fields i, j of a and b at stride fs)
+__global__ void k_stress_prod(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ out,
+                              long long fs, int i, int j) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < fs; e += (long long)gridDim.x * blockDim.x)
+        out[e] = a[i * fs + e] * a[j * fs + e] - b[i * fs + e] * b[j * fs + e];
+}
+
+// U_i += dt P(-i mask k_j tau_ij) for the velocity of a solver (tau: 6 spectral
+// fields xx xy xz yy yz zz at stride ntc); P the projection on k-perpendicular
+__global__ __launch_bounds__(256) void k_sgs_update(cpx<double> *__restrict__ U, const cpx<double> *__restrict__ T,
+                                                    long long ntc, KGeom kg, const TimeState *__restrict__ ts) {
+    const double dt = ts->dt;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long rowi = idx / kg.nzh;
+        const int k = (int)(idx - rowi * kg.nzh);
+        const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
+        double kx, ky, kz, m;
+        kvec_at(kg, i, j, k, kx, ky, kz, m);
+        const cpx<double> t0 = T[idx], t1 = T[ntc + idx], t2 = T[2 * ntc + idx], t3 = T[3 * ntc + idx],
+                          t4 = T[4 * ntc + idx], t5 = T[5 * ntc + idx];
+        const cpx<double> d0 = kdot3(kx, t0, ky, t1, kz, t2);
+        const cpx<double> d1 = kdot3(kx, t1, ky, t3, kz, t4);
+        const cpx<double> d2 = kdot3(kx, t2, ky, t4, kz, t5);
+        // -i m d
+        cpx<double> g[3] = {{m * d0.y, -m * d0.x}, {m * d1.y, -m * d1.x}, {m * d2.y, -m * d2.x}};
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 > 0.0) {
+            const cpx<double> kg_ = kdot3(kx, g[0], ky, g[1], kz, g[2]);
+            const double kk[3] = {kx, ky, kz};
+#pragma unroll
+            for (int q = 0; q < 3; q++) g[q] = cpx<double>{g[q].x - kk[q] * kg_.x / k2, g[q].y - kk[q] * kg_.y / k2};
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            cpx<double> &u = U[q * ntc + idx];
+            u = cpx<double>{u.x + dt * g[q].x, u.y + dt * g[q].y};
+        }
+    }
+}
+
 }  // namespace
 }  // namespace spk
 
@@ -385,6 +477,8 @@ struct spk_closure_s {
     bf16 *xin = nullptr, *Z[3] = {}, *A[3] = {}, *dzA = nullptr, *dzB = nullptr;
     float *out = nullptr, *tgt = nullptr, *part = nullptr, *bpart = nullptr, *lpart = nullptr;
     int adam_t = 0;
+    double *tau = nullptr;          // coupling: padded real stresses [6][n][n][n + 2], transformed in place
+    spk_fft_plan plan6 = nullptr;   // FP64 n^3 transforms of 6 fields
 };
 
 namespace {
@@ -426,7 +520,9 @@ int closure_alloc(spk_closure c) {
 }
 
 void closure_free(spk_closure c) {
-    void *b[] = {c->prm, c->grd, c->am, c->av, c->xin, c->dzA, c->dzB, c->out, c->tgt, c->part, c->bpart, c->lpart};
+    if (c->plan6) spk_fft_plan_destroy(c->plan6);
+    void *b[] = {c->prm, c->grd, c->am, c->av, c->xin, c->dzA, c->dzB, c->out, c->tgt, c->part, c->bpart, c->lpart,
+                 c->tau};
     for (void *p : b)
         if (p) (void)hipFree(p);
     for (int l = 0; l < 4; l++) {
@@ -625,4 +721,91 @@ extern "C" int spk_closure_train_step(spk_closure c, const void *u, const void *
                                                          c1, c2);
     SPK_LAUNCH_CHECK();
     return pack_weights(c);
+}
+
+// SGS term of a solver's state after its step: U += dt P(-i m k_j tau_ij(u))
+extern "C" int spk_closure_apply(spk_closure c, spk_solver s) {
+    SPK_REQUIRE(c && s, SPK_ERR_ARG, "spk_closure_apply: null argument");
+    SolverViews v;
+    solver_views(s, &v);
+    SPK_REQUIRE(v.incompressible && v.nx == c->n && v.ny == c->n && v.nz == c->n, SPK_ERR_ARG,
+                "spk_closure_apply: needs an incompressible run on the closure's %d^3 grid", c->n);
+    SPK_REQUIRE(v.ctx->stream == c->ctx->stream, SPK_ERR_ARG,
+                "spk_closure_apply: the closure and the solver must share a stream");
+    const int n = c->n;
+    const size_t fs = (size_t)n * n * (n + 2);
+    if (!c->tau) {
+        if (hipMalloc((void **)&c->tau, fs * 6 * sizeof(double)) != hipSuccess) {
+            c->tau = nullptr;
+            set_error("spk_closure_apply: hipMalloc of the stress buffer failed");
+            return SPK_ERR_ALLOC;
+        }
+        SPK_TRY(spk_fft_plan_create(c->ctx, n, n, n, SPK_F64, 6, &c->plan6));
+    }
+    SPK_TRY(spk_complex2real(s));
+    SPK_TRY(forward(c, v.realbuf + (size_t)v.VX * fs, false));
+    const double *sc = c->out_scale;
+    k_unpack_output<<<4096, 256, 0, c->ctx->stream>>>(c->out, c->tau, n, sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]);
+    SPK_LAUNCH_CHECK();
+    SPK_TRY(spk_fft_r2c(c->plan6, c->tau, c->tau));
+    k_sgs_update<<<4096, 256, 0, c->ctx->stream>>>(v.U + (long long)v.VX * v.ntc,
+                                                   reinterpret_cast<const cpx<double> *>(c->tau), v.ntc, v.kg, v.ts);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+// Filtered velocity and subgrid stress on an n^3 grid from a fine N^3 state:
+// with F the sharp cut to |k_i| <= kc, kc = (2/3)(n/2 - 1) (the dealiasing
+// cut of the coarse grid, wavevector.cu:70-96), ubar = F u and
+// tau_ij = F(u_i u_j) - F(ubar_i ubar_j), both products formed on the fine grid.
+extern "C" int spk_sgs_stress(spk_ctx ctx, const void *U_hi, int N, void *ubar_lo, void *tau_lo, int n) {
+    SPK_REQUIRE(ctx && U_hi && ubar_lo && tau_lo, SPK_ERR_ARG, "spk_sgs_stress: null argument");
+    SPK_REQUIRE(n >= 4 && N >= n && N % n == 0, SPK_ERR_ARG, "spk_sgs_stress: N = %d, n = %d", N, n);
+    hipStream_t st = ctx->stream;
+    const long long ntc_hi = (long long)N * N * (N / 2 + 1), ntc_lo = (long long)n * n * (n / 2 + 1);
+    const int kc = (int)(2.0 / 3.0 * (n / 2 - 1));
+    const double scale = std::pow((double)n / N, 3);
+    double *ur = nullptr, *br = nullptr, *pr = nullptr;
+    spk_fft_plan hi3 = nullptr, hi1 = nullptr, lo3 = nullptr, lo1 = nullptr;
+    int rc = SPK_OK;
+    auto run = [&]() -> int {
+        SPK_HIP(hipMalloc((void **)&ur, sizeof(double) * 2 * ntc_hi * 3));
+        SPK_HIP(hipMalloc((void **)&br, sizeof(double) * 2 * ntc_hi * 3));
+        SPK_HIP(hipMalloc((void **)&pr, sizeof(double) * 2 * ntc_hi));
+        SPK_TRY(spk_fft_plan_create(ctx, N, N, N, SPK_F64, 3, &hi3));
+        SPK_TRY(spk_fft_plan_create(ctx, N, N, N, SPK_F64, 1, &hi1));
+        SPK_TRY(spk_fft_plan_create(ctx, n, n, n, SPK_F64, 3, &lo3));
+        SPK_TRY(spk_fft_plan_create(ctx, n, n, n, SPK_F64, 1, &lo1));
+        const cpx<double> *U = reinterpret_cast<const cpx<double> *>(U_hi);
+        // fine real u, and fine real F u
+        SPK_TRY(spk_fft_c2r(hi3, U, ur));
+        k_mask_fine<<<4096, 256, 0, st>>>(U, reinterpret_cast<cpx<double> *>(br), N, ntc_hi, 3, kc);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(spk_fft_c2r(hi3, br, br));
+        // coarse F u
+        k_truncate<<<4096, 256, 0, st>>>(U, N, ntc_hi, reinterpret_cast<cpx<double> *>(ubar_lo), n, ntc_lo, 3, kc,
+                                         scale);
+        SPK_LAUNCH_CHECK();
+        SPK_TRY(spk_fft_c2r(lo3, ubar_lo, ubar_lo));
+        const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+        for (int c = 0; c < 6; c++) {
+            k_stress_prod<<<4096, 256, 0, st>>>(ur, br, pr, 2 * ntc_hi, I[c], J[c]);
+            SPK_LAUNCH_CHECK();
+            SPK_TRY(spk_fft_r2c(hi1, pr, pr));
+            double *t = (double *)tau_lo + (size_t)c * 2 * ntc_lo;
+            k_truncate<<<4096, 256, 0, st>>>(reinterpret_cast<const cpx<double> *>(pr), N, ntc_hi,
+                                             reinterpret_cast<cpx<double> *>(t), n, ntc_lo, 1, kc, scale);
+            SPK_LAUNCH_CHECK();
+            SPK_TRY(spk_fft_c2r(lo1, t, t));
+        }
+        SPK_HIP(hipStreamSynchronize(st));
+        return SPK_OK;
+    };
+    rc = run();
+    (void)hipStreamSynchronize(st);
+    for (spk_fft_plan pl : {hi3, hi1, lo3, lo1})
+        if (pl) spk_fft_plan_destroy(pl);
+    for (double *b : {ur, br, pr})
+        if (b) (void)hipFree(b);
+    return rc;
 }

@@ -23,6 +23,20 @@ struct KGeom {
     double kxshift;          // tremap * shear: sheared kx = kx0 + kxshift * ky (ShearWavevector)
 };
 
+// what a module outside the solver needs of it (the closure coupling)
+struct SolverViews {
+    cpx<double> *U;
+    double *realbuf;
+    TimeState *ts;
+    KGeom kg;
+    int VX;
+    long long ntc;
+    int nx, ny, nz;
+    spk_ctx ctx;
+    bool incompressible;
+};
+void solver_views(spk_solver s, SolverViews *v);
+
 __device__ __forceinline__ void kvec_at(const KGeom &g, int i, int j, int k, double &kx, double &ky,
                                         double &kz, double &m) {
     const int ii = i < g.nx / 2 ? i : i - g.nx;  // fmod(i + NX/2, NX) - NX/2

@@ -606,6 +606,19 @@ __global__ __launch_bounds__(256) void k_diag_sop(const cpx<double> *X, cpx<doub
     }
 }
 
+// Large-scale forcing: U[v0 + f][idx[m]] += dt F[m][f] for the listed modes
+__global__ void k_force(cpx<double> *__restrict__ U, long long ntc, int v0, const long long *__restrict__ idx,
+                        const cpx<double> *__restrict__ F, int nmodes, const TimeState *__restrict__ ts) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmodes) return;
+    const double dt = ts->dt;
+    for (int f = 0; f < 3; f++) {
+        cpx<double> &u = U[(long long)(v0 + f) * ntc + idx[m]];
+        const cpx<double> a = F[3 * m + f];
+        u = cpx<double>{u.x + dt * a.x, u.y + dt * a.y};
+    }
+}
+
 // UnshearComplexVec (cuda_kernels_generic.cu:425-439) on a copy of nf fields:
 // Z = U exp(-i ky c / 2), ky unsheared
 __global__ __launch_bounds__(256) void k_unshear(const cpx<double> *__restrict__ U, cpx<double> *__restrict__ Z,
@@ -664,6 +677,9 @@ struct spk_solver_s {
     int rkl_device;     // RKL2 cycles planned on the device (no host synchronization per cycle)
     int rkl_smax;       // stage count at the sts cap: the stages a device-planned cycle launches
     RklPlan *rkl_plan;  // device
+    int force_n;               // large-scale forcing: modes, their state indices and values (device)
+    long long *force_idx;
+    cpx<double> *force_val;
     InvProg inv_rkl_b, inv_rkl_g;
     cpx<double> *rUc0, *rUc1, *rdU0;
     TimeState *ts;
@@ -1238,8 +1254,9 @@ int step_fused(spk_solver s, bool w_valid) {
 }
 
 void free_all(spk_solver s) {
-    void *bufs[] = {s->U,    s->S,    s->dU,   s->realbuf, s->W,        s->P,  s->W2,   s->P2,   s->Breal, s->diag,
-                    s->rUc0, s->rUc1, s->rdU0, s->rkl_plan, s->ts, s->pmax, s->part, s->res,  s->spec};
+    void *bufs[] = {s->U,    s->S,    s->dU,   s->realbuf,  s->W,  s->P,    s->W2,   s->P2,  s->Breal,     s->diag,
+                    s->rUc0, s->rUc1, s->rdU0, s->rkl_plan, s->ts, s->pmax, s->part, s->res, s->spec, s->force_idx,
+                    s->force_val};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     twiddles_destroy(&s->twb);
@@ -1560,14 +1577,47 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
     const bool fused = fused_ok(s);
     for (int n = 0; n < nsteps; n++) {
         if (fused) {
-            SPK_TRY(step_fused(s, n > 0));
+            // forcing changes the state after the fused back end made the next step's
+            // inverse x pass, so that pass is made again from the state
+            SPK_TRY(step_fused(s, n > 0 && s->force_n == 0));
         } else {
             SPK_TRY(clean_divergence(s));
             s->products_valid = false;
             SPK_TRY(hydro_mhd_advance(s));
         }
+        if (s->force_n > 0) {
+            k_force<<<(s->force_n + 255) / 256, 256, 0, s->ctx->stream>>>(s->U, s->g.ntc, s->VX, s->force_idx,
+                                                                          s->force_val, s->force_n, s->ts);
+            SPK_LAUNCH_CHECK();
+        }
         SPK_TRY(clock(s, 0, 0, 1, 0, 0.0));
     }
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_set_forcing(spk_solver s, int nmodes, const long long *index, const double *values) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(nmodes >= 0 && (nmodes == 0 || (index && values)), SPK_ERR_ARG, "spk_solver_set_forcing: bad arguments");
+    SPK_REQUIRE(nmodes == 0 || (s->p.incompressible && !s->p.heat_equation), SPK_ERR_ARG,
+                "spk_solver_set_forcing: forcing acts on the velocity of an incompressible run");
+    for (int m = 0; m < nmodes; m++)
+        SPK_REQUIRE(index[m] >= 0 && index[m] < s->g.ntc, SPK_ERR_ARG, "spk_solver_set_forcing: mode index %lld",
+                    index[m]);
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (s->force_idx) (void)hipFree(s->force_idx);
+    if (s->force_val) (void)hipFree(s->force_val);
+    s->force_idx = nullptr;
+    s->force_val = nullptr;
+    s->force_n = 0;
+    if (nmodes == 0) return SPK_OK;
+    SPK_TRY(dalloc(&s->force_idx, sizeof(long long) * (size_t)nmodes));
+    SPK_TRY(dalloc(&s->force_val, sizeof(cpx<double>) * 3 * (size_t)nmodes));
+    SPK_HIP(hipMemcpyAsync(s->force_idx, index, sizeof(long long) * (size_t)nmodes, hipMemcpyHostToDevice,
+                           s->ctx->stream));
+    SPK_HIP(hipMemcpyAsync(s->force_val, values, sizeof(double) * 6 * (size_t)nmodes, hipMemcpyHostToDevice,
+                           s->ctx->stream));
+    SPK_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->force_n = nmodes;
     return SPK_OK;
 }
 
@@ -2086,6 +2136,19 @@ extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int ca
 }
 
 namespace spk {
+void solver_views(spk_solver s, SolverViews *v) {
+    v->U = s->U;
+    v->realbuf = s->realbuf;
+    v->ts = s->ts;
+    v->kg = kgeom(s, s->tremap * s->p.shear);
+    v->VX = s->VX;
+    v->ntc = s->g.ntc;
+    v->nx = s->g.nx;
+    v->ny = s->g.ny;
+    v->nz = s->g.nz;
+    v->ctx = s->ctx;
+    v->incompressible = s->p.incompressible && !s->p.heat_equation;
+}
 spk_ctx solver_ctx(spk_solver s) { return s->ctx; }
 int solver_host_step(spk_solver s) { return s->step; }
 // host-side step count after n steps that ran from a replayed graph

@@ -238,3 +238,95 @@ def test_training_fits_a_teacher(ctx):
     assert np.isfinite(last) and last < 0.5 * first, (first, last)
     teacher.destroy()
     student.destroy()
+
+
+ini = __import__("importlib").import_module("spooky-git_amd.initial")
+TWO_PI = 2 * np.pi
+
+
+def hd_params(n, nu=0.01):
+    return dict(nx=n, ny=n, nz=n, incompressible=1, nu=nu, cfl=0.5, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI,
+                t_final=1e9, toutput_flow=1e9)
+
+
+def test_forcing_single_shear_mode(ctx):
+    """U += dt F after each step: on a single shear mode (no nonlinear term) the
+    state follows x_{n+1} = R(-nu k^2 dt_n) x_n + dt_n F, R(z) = 1 + z + z^2/2 + z^3/6
+    (the RK3 amplification), with the run's own dt_n."""
+    n, nu = 32, 0.05
+    sol = spk.Solver(ctx, "f64", **hd_params(n, nu))
+    nzh = n // 2 + 1
+    i1 = (0 * n + 1) * nzh          # (kx, ky, kz) = (0, 1, 0)
+    i2 = (0 * n + (n - 1)) * nzh    # (0, -1, 0), its Hermitian partner on kz = 0
+    c = (0.3 + 0.2j) * n ** 3
+    sol.set_forcing([i1, i2], [[c, 0, 0], [np.conj(c), 0, 0]])
+    x = 0.0 + 0.0j
+    for _ in range(20):
+        sol.main_loop(1)
+        dt = sol.time()["dt"]
+        z = -nu * dt
+        x = (1 + z + z * z / 2 + z ** 3 / 6) * x + dt * c
+    U = sol.get_fields()
+    assert abs(U[0].reshape(-1)[i1] - x) < 1e-12 * abs(x)
+    assert abs(U[0].reshape(-1)[i2] - np.conj(x)) < 1e-12 * abs(x)
+    U[0].reshape(-1)[[i1, i2]] = 0
+    assert np.max(np.abs(U)) < 1e-12 * abs(x)
+    with pytest.raises(RuntimeError):
+        sol.set_forcing([n ** 3], [[1, 0, 0]])
+    sol.destroy()
+
+
+def test_sgs_stress_two_modes(ctx):
+    """u_x = cos(3 y) + cos(12 y) on 64^3, coarse grid 32^3 (cut kc = 10): ubar_x =
+    cos(3 y) and tau_xx = F(u_x^2) - F(ubar_x^2) = 1/2 + cos(9 y); the other
+    components vanish."""
+    N, n = 64, 32
+    X, Y, Z = ini.grid_coords(N, N, N, TWO_PI, TWO_PI, TWO_PI)
+    u = np.stack([np.cos(3 * Y) + np.cos(12 * Y), 0 * Y, 0 * Y])
+    U = np.stack([np.fft.rfftn(f) for f in u])
+    dU = device_fields(ctx, U)
+    dub = ctx.alloc(3 * n * n * (n + 2) * 8)
+    dtau = ctx.alloc(6 * n * n * (n + 2) * 8)
+    spk.sgs_stress(ctx, dU.ptr, N, dub.ptr, dtau.ptr, n)
+    ub = dub.download(np.float64, (3, n, n, n + 2))[..., :n]
+    tau = dtau.download(np.float64, (6, n, n, n + 2))[..., :n]
+    x, y, z = ini.grid_coords(n, n, n, TWO_PI, TWO_PI, TWO_PI)
+    assert np.max(np.abs(ub[0] - np.cos(3 * y))) < 1e-12 and np.max(np.abs(ub[1:])) < 1e-12
+    assert np.max(np.abs(tau[0] - (0.5 + np.cos(9 * y)))) < 1e-12
+    assert np.max(np.abs(tau[1:])) < 1e-12
+
+
+def test_closure_apply_adds_the_projected_divergence(ctx):
+    n = 32
+    rng = np.random.default_rng(4)
+    p = hd_params(n)
+    sol = spk.Solver(ctx, "f64", **p)
+    sol.set_fields(0.5 * ini.random_solenoidal((n, n, n), (TWO_PI,) * 3, seed=9, k0=3.0))
+    sol.main_loop(1)
+    cl = spk.Closure(ctx, n)
+    cl.set_params(random_params(rng))
+    scales = [0.1, 0.05, 0.05, 0.1, 0.05, 0.1]
+    cl.set_scales(2.0, scales)
+    before = sol.get_fields()
+    dt = sol.time()["dt"]
+    # the stress the network gives on this state
+    sol.complex2real()
+    dtau = ctx.alloc(6 * n * n * (n + 2) * 8)
+    cl.forward(sol.d_realbuf.value, dtau.ptr)
+    tau = dtau.download(np.float64, (6, n, n, n + 2))[..., :n]
+    cl.apply(sol)
+    after = sol.get_fields()
+    T = [np.fft.rfftn(t) for t in tau]
+    KX, KY, KZ = ini.wavenumbers(n, n, n, TWO_PI, TWO_PI, TWO_PI)
+    m = ini.dealias_mask(n, n, n, TWO_PI, TWO_PI, TWO_PI)
+    d = [KX * T[0] + KY * T[1] + KZ * T[2], KX * T[1] + KY * T[3] + KZ * T[4], KX * T[2] + KY * T[4] + KZ * T[5]]
+    g = [-1j * m * di for di in d]
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    kg = KX * g[0] + KY * g[1] + KZ * g[2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        g = [np.where(K2 > 0, gi - k * kg / np.where(K2 > 0, K2, 1), gi) for gi, k in zip(g, (KX, KY, KZ))]
+    expect = before + dt * np.stack(g)
+    assert np.max(np.abs(after - expect)) < 1e-10 * np.max(np.abs(after - before))
+    assert np.max(np.abs(after - before)) > 0
+    cl.destroy()
+    sol.destroy()
