@@ -53,7 +53,10 @@ def parse():
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the Taylor-Green 64^3 and MTI legs")
     ap.add_argument("--ensemble-steps", type=int, default=20)
-    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble"], default="full",
+    ap.add_argument("--closure-spinup", type=float, default=1.0)
+    ap.add_argument("--closure-iters", type=int, default=2000)
+    ap.add_argument("--closure-dns", type=int, default=512)
+    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure"], default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
                          "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs); "
                          "mti: only the MTI leg (--mti-steps steps)")
@@ -386,6 +389,180 @@ def ensemble_leg(spk, ini, ctx, steps):
                 params="nu, eta log-uniform [1e-4, 1e-2]; chi_par uniform [1e-3, 1e-1] (hbi); seeds 5000+i")
 
 
+def forced_modes(n, L=2 * 3.141592653589793, kf=2.0, f0=0.3, seed=11):
+    """Large-scale forcing field on an n^3 grid: a solenoidal random field on the
+    modes 0 < |k| <= kf (integer k in a 2 pi box), real-space rms f0, seeded so
+    that every grid gets the same physical field.  Returns (indices, values)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    kmax = int(kf)
+    ks = [(a, b, c) for a in range(-kmax, kmax + 1) for b in range(-kmax, kmax + 1) for c in range(0, kmax + 1)
+          if 0 < a * a + b * b + c * c <= kf * kf]
+    nzh = n // 2 + 1
+    F = np.zeros((3, n, n, nzh), np.complex128)
+    for (a, b, c) in ks:
+        v = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+        k = np.array([a, b, c], float)
+        v -= k * (k @ v) / (k @ k)
+        F[:, a % n, b % n, c] = v
+    r = np.stack([np.fft.irfftn(f, s=(n, n, n), axes=(0, 1, 2)) for f in F])  # Hermitian-consistent, real
+    r *= f0 / np.sqrt(np.mean(np.sum(r ** 2, axis=0)))
+    F = np.stack([np.fft.rfftn(x) for x in r])
+    idx = np.array([((a % n) * n + (b % n)) * nzh + c for (a, b, c) in ks], dtype=np.int64)
+    idx = np.unique(np.concatenate([idx, [((-a % n) * n + (-b % n)) * nzh for (a, b, c) in ks if c == 0]]))
+    flat = F.reshape(3, -1)
+    return idx, flat[:, idx].T.copy()
+
+
+def shell_energy(sol):
+    """E(k) of the velocity (sum of the three component spectra)."""
+    v = sol.vars
+    return sum(sol.spectrum(v["VX"] + i, v["VX"] + i) for i in range(3))
+
+
+def closure_leg(spk, ini, ctx, torch, N=512, t_spin=1.0, nsamples=6, dt_sample=0.05, dT=0.5, iters=2000,
+                lr=1e-3):
+    """Learned subgrid closure for forced HD turbulence: a 512^3 FP32-bulk DNS
+    (nu = 2e-4, forcing at |k| <= 2, random-phase E(k) ~ k^4 exp(-2 (k/4)^2)
+    start) gives filtered velocity / subgrid stress pairs on 128^3; the network is
+    trained on them; then from the filtered DNS state the 128^3 run with the
+    closure, the plain 128^3 run and the DNS advance by dT, and their shell
+    spectra and wall times are compared (bins 1 .. kc/2, kc = the 128^3 cut)."""
+    import numpy as np
+    T = 2 * np.pi
+    n = N // 4
+    t0 = t_spin + (nsamples - 1) * dt_sample
+    t_end = t0 + dT
+    base = dict(incompressible=1, nu=2e-4, cfl=0.5, lx=T, ly=T, lz=T, toutput_flow=1e9)
+    dns = spk.Solver(ctx, "f32", **dict(base, nx=N, ny=N, nz=N, t_final=t_end))
+    dns.set_fields(ini.random_solenoidal((N, N, N), (T, T, T), seed=2027, k0=4.0))
+    idx, val = forced_modes(N)
+    dns.set_forcing(idx, val)
+    res = dict(dns_grid=N, les_grid=n, nu=2e-4, forcing="solenoidal random field on 0<|k|<=2, rms 0.3, fixed",
+               ic="random-phase solenoidal, E(k)~k^4 exp(-2(k/4)^2), rms 1, seed 2027")
+
+    def run_until(sol, t, after=None):
+        steps, last = 0, time.perf_counter()
+        while sol.time()["t"] < t - 1e-12:
+            sol.main_loop(1)
+            if after:
+                after(sol)
+            steps += 1
+            if time.perf_counter() - last > 30:
+                last = time.perf_counter()
+                print("# closure: t = %.4f after %d steps" % (sol.time()["t"], steps), file=sys.stderr, flush=True)
+        return steps
+
+    def clock():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    tic = clock()
+    res["spinup_steps"] = run_until(dns, t_spin)
+    res["spinup_s"] = round(clock() - tic, 2)
+    print("# closure: DNS %d^3 spun up to t=%.2f in %d steps (%.1f s)" % (N, t_spin, res["spinup_steps"],
+          res["spinup_s"]), file=sys.stderr, flush=True)
+    fs = n * n * (n + 2)
+    samples = []
+    for s in range(nsamples):
+        if s:
+            run_until(dns, t_spin + s * dt_sample)
+        ub, tau = ctx.alloc(3 * fs * 8), ctx.alloc(6 * fs * 8)
+        spk.sgs_stress(ctx, dns.d_fields.value, N, ub.ptr, tau.ptr, n)
+        samples.append((ub, tau))
+    ubh = [u.download(np.float64, (3, n, n, n + 2))[..., :n] for u, _ in samples]
+    tah = [t.download(np.float64, (6, n, n, n + 2))[..., :n] for _, t in samples]
+    u_rms = float(np.sqrt(np.mean([np.mean(np.sum(u ** 2, 0)) for u in ubh])))
+    tau_rms = [float(np.sqrt(np.mean([np.mean(t[c] ** 2) for t in tah]))) for c in range(6)]
+    cl = spk.Closure(ctx, n)
+    rng = np.random.default_rng(7)
+    p0 = np.zeros(cl.nparams, np.float32)
+    for l, ((ws, bsl), cin) in enumerate(zip(spk.Closure.layer_slices(), spk.Closure.CIN)):
+        w = rng.standard_normal((27, 32, cin)) / np.sqrt(27 * (3 if cin == 16 else cin))
+        if cin == 16:
+            w[..., 3:] = 0
+        if l == 3:
+            w[:, 6:, :] = 0
+            w *= 0.1
+        p0[ws] = w.reshape(-1)
+    cl.set_params(p0)
+    cl.set_scales(1.0 / u_rms, tau_rms)
+    curve = []
+    train_s = 0.0
+    first = None
+    for it in range(iters):
+        rate = lr if it < 0.6 * iters else 0.3 * lr
+        ub, tau = samples[it % (nsamples - 1)]  # the last sample is held out
+        tic = clock()
+        loss = cl.train_step(ub.ptr, tau.ptr, rate, want_loss=(it == 0))
+        train_s += clock() - tic
+        if it == 0:
+            first = loss
+        if (it + 1) % max(1, iters // 8) == 0 or it == iters - 1:
+            tr = float(np.mean([cl.grads(u_.ptr, t_.ptr) for u_, t_ in samples[:-1]]))
+            held = cl.grads(samples[-1][0].ptr, samples[-1][1].ptr)
+            curve.append([it + 1, round(tr, 4), round(held, 4)])
+            print("# closure: iteration %d: training loss %.4f, held-out %.4f" % (it + 1, tr, held),
+                  file=sys.stderr, flush=True)
+    last, held = curve[-1][1], curve[-1][2]
+    res.update(u_rms_filtered=u_rms, tau_rms=tau_rms, train_iters=iters, train_s=round(train_s, 2),
+               train_loss_first=first, train_loss_last=last, heldout_loss=held, loss_curve=curve,
+               ms_per_train_iter=round(1000 * train_s / iters, 3))
+    print("# closure: trained %d iterations in %.1f s, loss %.3f -> %.3f, held-out %.3f" %
+          (iters, train_s, first, last, held), file=sys.stderr, flush=True)
+    # inference time on 128^3
+    dtau = ctx.alloc(6 * fs * 8)
+    cl.forward(samples[-1][0].ptr, dtau.ptr)
+    tic = clock()
+    for _ in range(10):
+        cl.forward(samples[-1][0].ptr, dtau.ptr)
+    res["ms_per_inference"] = round((clock() - tic) * 100, 3)
+    # a posteriori from the filtered DNS state at t0
+    les_p = dict(base, nx=n, ny=n, nz=n, t_final=t_end)
+    uhat0 = np.stack([np.fft.rfftn(u) for u in ubh[-1]])
+    lidx, lval = forced_modes(n)
+    runs = {}
+    for name, with_closure in (("plain", False), ("closure", True)):
+        les = spk.Solver(ctx, "f32", **les_p)
+        les.set_fields(uhat0)
+        les.set_time(t0)
+        les.set_forcing(lidx, lval)
+        tic = clock()
+        steps = run_until(les, t_end, (lambda s: cl.apply(s)) if with_closure else None)
+        wall = clock() - tic
+        E = shell_energy(les)
+        runs[name] = dict(steps=steps, wall_s=wall, E=E, finite=bool(np.all(np.isfinite(E))))
+        les.destroy()
+    tic = clock()
+    dns_steps = run_until(dns, t_end)
+    dns_wall = clock() - tic
+    # filtered DNS spectrum at t_end on the 128^3 grid
+    ub, tau = ctx.alloc(3 * fs * 8), ctx.alloc(6 * fs * 8)
+    spk.sgs_stress(ctx, dns.d_fields.value, N, ub.ptr, tau.ptr, n)
+    fl = spk.Solver(ctx, "f64", **les_p)
+    fl.set_fields(np.stack([np.fft.rfftn(u) for u in ub.download(np.float64, (3, n, n, n + 2))[..., :n]]))
+    E_dns = shell_energy(fl)
+    fl.destroy()
+    kc = int(2.0 / 3.0 * (n / 2 - 1))
+    bins = np.arange(1, kc // 2 + 1)
+    out = {}
+    for name, r in runs.items():
+        err = np.abs(r["E"][bins] / E_dns[bins] - 1.0) if r["finite"] else np.full(len(bins), np.inf)
+        out[name] = dict(steps=r["steps"], wall_s=round(r["wall_s"], 3), max_rel_spectrum_err=float(np.max(err)),
+                         mean_rel_spectrum_err=float(np.mean(err)), finite=r["finite"],
+                         speedup_vs_dns=round(dns_wall / r["wall_s"], 1))
+        print("# closure: %s %d^3: %d steps %.2f s, spectrum error max %.3f mean %.3f (bins 1..%d)" %
+              (name, n, r["steps"], r["wall_s"], float(np.max(err)), float(np.mean(err)), kc // 2),
+              file=sys.stderr, flush=True)
+    res.update(t0=t0, dT=dT, dns_steps_dT=dns_steps, dns_wall_s_dT=round(dns_wall, 2), compare_bins=[1, kc // 2],
+               les=out, E_dns_filtered=[float(x) for x in E_dns[: kc + 1]],
+               E_les_closure=[float(x) for x in runs["closure"]["E"][: kc + 1]],
+               E_les_plain=[float(x) for x in runs["plain"]["E"][: kc + 1]])
+    dns.destroy()
+    cl.destroy()
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -410,6 +587,10 @@ def main():
         return
     if a.mode == "ensemble":
         print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps)))
+        return
+    if a.mode == "closure":
+        print(json.dumps(closure_leg(spk, ini, ctx, torch, N=a.closure_dns, t_spin=a.closure_spinup,
+                                     iters=a.closure_iters)))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
     if a.mode == "kernels":
