@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--closure-spinup", type=float, default=1.0)
     ap.add_argument("--closure-iters", type=int, default=2000)
     ap.add_argument("--closure-dns", type=int, default=512)
-    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure"], default="full",
+    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure", "closure-kernels"],
+                    default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
                          "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs); "
                          "mti: only the MTI leg (--mti-steps steps)")
@@ -563,6 +564,44 @@ def closure_leg(spk, ini, ctx, torch, N=512, t_spin=1.0, nsamples=6, dt_sample=0
     return res
 
 
+def closure_kernels(spk, ctx, torch, n=128, reps=20):
+    """The closure's kernels alone on a 128^3 grid: inference and training
+    iterations timed with HIP events, and the matrix-core rate of one inference
+    (algorithmic flops: 2 P 27 (3*32 + 32*32 + 32*32 + 32*6))."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    cl = spk.Closure(ctx, n)
+    p = (rng.standard_normal(cl.nparams) * 0.05).astype(np.float32)
+    cl.set_params(p)
+    cl.set_scales(1.0, [1.0] * 6)
+    fs = n * n * (n + 2)
+    u = ctx.alloc(3 * fs * 8)
+    u.upload(rng.standard_normal(3 * fs))
+    tau = ctx.alloc(6 * fs * 8)
+    tau.upload(rng.standard_normal(6 * fs) * 0.1)
+    cl.forward(u.ptr, tau.ptr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        cl.forward(u.ptr, tau.ptr)
+    e1.record()
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    cl.train_step(u.ptr, tau.ptr, 1e-4, want_loss=False)
+    e0.record()
+    for _ in range(reps):
+        cl.train_step(u.ptr, tau.ptr, 1e-4, want_loss=False)
+    e1.record()
+    torch.cuda.synchronize()
+    tr_ms = e0.elapsed_time(e1) / reps
+    P = n ** 3
+    flops = 2.0 * P * 27 * (3 * 32 + 32 * 32 + 32 * 32 + 32 * 6)
+    cl.destroy()
+    return dict(grid=n, inference_ms=round(inf_ms, 4), train_iter_ms=round(tr_ms, 4),
+                inference_tflops=round(flops / inf_ms / 1e9, 1), inference_frac_of_bf16_peak=round(
+                    flops / inf_ms / 1e9 / 2500.0, 4))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -587,6 +626,9 @@ def main():
         return
     if a.mode == "ensemble":
         print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps)))
+        return
+    if a.mode == "closure-kernels":
+        print(json.dumps(closure_kernels(spk, ctx, torch)))
         return
     if a.mode == "closure":
         print(json.dumps(closure_leg(spk, ini, ctx, torch, N=a.closure_dns, t_spin=a.closure_spinup,

@@ -56,19 +56,22 @@ constexpr size_t conv_lds(int cin) { return (size_t)(NTAP * CH * cin + HX * HY *
 // MODE 0: hidden layer, Y = GELU(conv + b); 1: also Z = conv + b (training);
 // 2: output layer, out[c][p] = conv + b for c < 6 (FP32, channel-major);
 // 3: input gradient, Y = conv(dZ, flipped transposed W) * GELU'(Zp)
+constexpr int CONV_T = 512;  // 8 waves: two per SIMD, so one wave's LDS reads overlap the other's MFMAs
+
 template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
-                                              const float *__restrict__ bias, bf16 *__restrict__ Y,
-                                              bf16 *__restrict__ Z, const bf16 *__restrict__ Zp,
-                                              float *__restrict__ out, int n) {
+__global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                                 const float *__restrict__ bias, bf16 *__restrict__ Y,
+                                                 bf16 *__restrict__ Z, const bf16 *__restrict__ Zp,
+                                                 float *__restrict__ out, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     bf16 *Ws = reinterpret_cast<bf16 *>(smem);  // [27][32][CIN], chunk-swizzled by co
     bf16 *Xs = Ws + NTAP * CH * CIN;            // [HX*HY*HZ][CIN], chunk-swizzled by halo point
     constexpr int NQ = CIN / 8;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 31, half = lane >> 5;
+    const int xw = w & 3, yg = 2 * (w >> 2);  // the wave's x plane and first of its two y columns
 
-    for (int i = tid; i < NTAP * CH * NQ; i += 256) {
+    for (int i = tid; i < NTAP * CH * NQ; i += CONV_T) {
         const int q = i % NQ, row = i / NQ;
         const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + (size_t)row * CIN + 8 * q);
         *reinterpret_cast<bf16x8 *>(Ws + row * CIN + 8 * swz(q, row & 31, CIN)) = v;
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const 
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
         __syncthreads();
-        for (int i = tid; i < HX * HY * HZ * NQ; i += 256) {
+        for (int i = tid; i < HX * HY * HZ * NQ; i += CONV_T) {
             const int q = i % NQ, hp = i / NQ;
             const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
             const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
@@ -89,11 +92,12 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const 
                 *reinterpret_cast<const bf16x8 *>(X + g);
         }
         __syncthreads();
-        f32x16 acc[4];
+        f32x16 acc[2];
 #pragma unroll
-        for (int ys = 0; ys < 4; ys++)
+        for (int ys = 0; ys < 2; ys++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[ys][r] = 0.0f;
+#pragma unroll
         for (int t = 0; t < NTAP; t++) {
             const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
 #pragma unroll
@@ -101,8 +105,8 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const 
                 const int q = 2 * kc + half;
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Ws + (t * CH + li) * CIN + 8 * swz(q, li, CIN));
 #pragma unroll
-                for (int ys = 0; ys < 4; ys++) {
-                    const int hp = ((w + 1 + dx) * HY + (ys + 1 + dy)) * HZ + li + 1 + dz;
+                for (int ys = 0; ys < 2; ys++) {
+                    const int hp = ((xw + 1 + dx) * HY + (yg + ys + 1 + dy)) * HZ + li + 1 + dz;
                     const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN));
                     acc[ys] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[ys], 0, 0, 0);
                 }
@@ -110,8 +114,8 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const 
         }
         // D[co][point]: lane holds point li, co = 8 (r / 4) + 4 half + r % 4
 #pragma unroll
-        for (int ys = 0; ys < 4; ys++) {
-            const size_t p = ((size_t)(x0 + w) * n + (y0 + ys)) * n + z0 + li;
+        for (int ys = 0; ys < 2; ys++) {
+            const size_t p = ((size_t)(x0 + xw) * n + (y0 + yg + ys)) * n + z0 + li;
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 const int c0 = 8 * g + 4 * half;
@@ -149,17 +153,17 @@ constexpr int XPL = HX * HY * 48 + 8;        // X^T plane (one ci): z rows of 48
 constexpr size_t DW_LDS = (size_t)(CH * DZS + CH * XPL) * sizeof(bf16);
 
 template <int CIN>
-__global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
+__global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
                                                  float *__restrict__ part, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     bf16 *Ds = reinterpret_cast<bf16 *>(smem);  // [32 co][DZS]: point (bx, by, z) at (bx 4 + by) 32 + z
     bf16 *Xs = Ds + CH * DZS;                   // [32 ci][XPL]: (hx, hy, z0 - 8 + e) at (hx 6 + hy) 48 + e
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 31, half = lane >> 5;
-    for (int i = tid; i < CH * XPL; i += 256) Xs[i] = (bf16)0.0f;  // padding and channels >= CIN stay 0
-    f32x16 acc[7];
+    for (int i = tid; i < CH * XPL; i += CONV_T) Xs[i] = (bf16)0.0f;  // padding and channels >= CIN stay 0
+    f32x16 acc[4];  // taps w, w + 8, w + 16, w + 24 of wave w
 #pragma unroll
-    for (int k = 0; k < 7; k++)
+    for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[k][r] = 0.0f;
     const int nbz = n / 32, nby = n / 4;
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, con
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
         __syncthreads();
-        for (int i = tid; i < 512 * 4; i += 256) {
+        for (int i = tid; i < 512 * 4; i += CONV_T) {
             const int q = i & 3, pt = i >> 2;
             const size_t p = ((size_t)(x0 + (pt >> 7)) * n + y0 + ((pt >> 5) & 3)) * n + z0 + (pt & 31);
             const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
@@ -176,7 +180,7 @@ __global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, con
             for (int j = 0; j < 8; j++) Ds[(8 * q + j) * DZS + pt] = v[j];
         }
         constexpr int NQ = CIN / 8;
-        for (int i = tid; i < HX * HY * HZ * NQ; i += 256) {
+        for (int i = tid; i < HX * HY * HZ * NQ; i += CONV_T) {
             const int q = i % NQ, hp = i / NQ;
             const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
             const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
@@ -192,8 +196,8 @@ __global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, con
             const int bxl = col >> 2, byl = col & 3;
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Ds + li * DZS + col * 32 + zr);
 #pragma unroll
-            for (int k = 0; k < 7; k++) {
-                const int t = w + 4 * k;
+            for (int k = 0; k < 4; k++) {
+                const int t = w + 8 * k;
                 if (t < NTAP) {
                     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
                     const int base = li * XPL + ((bxl + 1 + dx) * HY + (byl + 1 + dy)) * 48 + 8 + zr;
@@ -222,8 +226,8 @@ __global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X, con
     // D[co][ci]: lane holds ci = li, co = 8 (r / 4) + 4 half + r % 4
     float *pb = part + (size_t)blockIdx.x * NTAP * CH * CH;
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        const int t = w + 4 * k;
+    for (int k = 0; k < 4; k++) {
+        const int t = w + 8 * k;
         if (t < NTAP) {
 #pragma unroll
             for (int r = 0; r < 16; r++) {
@@ -573,15 +577,15 @@ int forward(spk_closure c, const void *u, bool train) {
     const float *b[4];
     for (int l = 0; l < 4; l++) b[l] = c->prm + param_offset(l) + w_count(l);
     if (train) {
-        k_conv<16, 1><<<g, 256, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], c->Z[0], nullptr, nullptr, c->n);
-        k_conv<32, 1><<<g, 256, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], c->Z[1], nullptr, nullptr, c->n);
-        k_conv<32, 1><<<g, 256, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], c->Z[2], nullptr, nullptr, c->n);
+        k_conv<16, 1><<<g, CONV_T, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], c->Z[0], nullptr, nullptr, c->n);
+        k_conv<32, 1><<<g, CONV_T, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], c->Z[1], nullptr, nullptr, c->n);
+        k_conv<32, 1><<<g, CONV_T, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], c->Z[2], nullptr, nullptr, c->n);
     } else {
-        k_conv<16, 0><<<g, 256, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], nullptr, nullptr, nullptr, c->n);
-        k_conv<32, 0><<<g, 256, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], nullptr, nullptr, nullptr, c->n);
-        k_conv<32, 0><<<g, 256, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], nullptr, nullptr, nullptr, c->n);
+        k_conv<16, 0><<<g, CONV_T, conv_lds(16), st>>>(c->xin, c->Wb[0], b[0], c->A[0], nullptr, nullptr, nullptr, c->n);
+        k_conv<32, 0><<<g, CONV_T, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], nullptr, nullptr, nullptr, c->n);
+        k_conv<32, 0><<<g, CONV_T, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], nullptr, nullptr, nullptr, c->n);
     }
-    k_conv<32, 2><<<g, 256, conv_lds(32), st>>>(c->A[2], c->Wb[3], b[3], nullptr, nullptr, nullptr, c->out, c->n);
+    k_conv<32, 2><<<g, CONV_T, conv_lds(32), st>>>(c->A[2], c->Wb[3], b[3], nullptr, nullptr, nullptr, c->out, c->n);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
@@ -590,8 +594,8 @@ int forward(spk_closure c, const void *u, bool train) {
 int layer_grads(spk_closure c, int l, const bf16 *X, const bf16 *dz) {
     hipStream_t st = c->ctx->stream;
     const unsigned g = conv_grid(c);
-    if (CIN_L[l] == 16) k_conv_dw<16><<<g, 256, DW_LDS, st>>>(X, dz, c->part, c->n);
-    else k_conv_dw<32><<<g, 256, DW_LDS, st>>>(X, dz, c->part, c->n);
+    if (CIN_L[l] == 16) k_conv_dw<16><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->n);
+    else k_conv_dw<32><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->n);
     const int cnt = NTAP * CH * CH;
     // the [27][32][32] sums land in grd for cin = 32; for cin = 16 take ci < 16
     float *gw = c->grd + param_offset(l);
@@ -693,11 +697,11 @@ extern "C" int spk_closure_grads(spk_closure c, const void *u, const void *tau_t
     const unsigned g = conv_grid(c);
     // layer 4, then back through layers 3, 2, 1
     SPK_TRY(layer_grads(c, 3, c->A[2], c->dzA));
-    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzA, c->WT[3], nullptr, c->dzB, nullptr, c->Z[2], nullptr, c->n);
+    k_conv<32, 3><<<g, CONV_T, conv_lds(32), st>>>(c->dzA, c->WT[3], nullptr, c->dzB, nullptr, c->Z[2], nullptr, c->n);
     SPK_TRY(layer_grads(c, 2, c->A[1], c->dzB));
-    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzB, c->WT[2], nullptr, c->dzA, nullptr, c->Z[1], nullptr, c->n);
+    k_conv<32, 3><<<g, CONV_T, conv_lds(32), st>>>(c->dzB, c->WT[2], nullptr, c->dzA, nullptr, c->Z[1], nullptr, c->n);
     SPK_TRY(layer_grads(c, 1, c->A[0], c->dzA));
-    k_conv<32, 3><<<g, 256, conv_lds(32), st>>>(c->dzA, c->WT[1], nullptr, c->dzB, nullptr, c->Z[0], nullptr, c->n);
+    k_conv<32, 3><<<g, CONV_T, conv_lds(32), st>>>(c->dzA, c->WT[1], nullptr, c->dzB, nullptr, c->Z[0], nullptr, c->n);
     SPK_TRY(layer_grads(c, 0, c->xin, c->dzB));
     SPK_LAUNCH_CHECK();
     if (loss) {
