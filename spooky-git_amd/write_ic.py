@@ -13,6 +13,8 @@ IC:
   tg3d           v = (sin x cos y cos z, -cos x sin y cos z, 0) with x scaled by 2 pi/L
   mhd-random     random-phase solenoidal u and B, E(k) ~ k^4 exp(-2 (k/k0)^2), rms 1 (--seed, --k0)
   mti-noise      u = 0, B = (b0, 0, 0), theta white noise uniform in [-A, A] (--seed, --amplitude, --b0)
+  hbi-noise      u_x, u_y random amplitude A U(-1/2, 1/2) sqrt(27/8 N) and random phase at every
+                 scale, B = (0, b0, 0), theta = 0 (problems/hbi2d; --seed, --amplitude, --b0)
 """
 import argparse
 import importlib
@@ -63,6 +65,11 @@ def real_fields(p, ic, seed=0, amplitude=1e-4, b0=0.0, k0=4.0):
     elif ic == "mti-noise":
         c = initial.mti_white_noise((nx, ny, nz), (lx, ly, lz), seed=seed, amp=amplitude, b0=b0)
         out["th"] = np.fft.irfftn(c[6], s=(nx, ny, nz))
+    elif ic == "hbi-noise":
+        c = initial.hbi_noise((nx, ny, nz), (lx, ly, lz), seed=seed, amp=amplitude, b0=b0)
+        for i, n in enumerate(("vx", "vy", "vz", "bx", "by", "bz", "th")):
+            if n in out:
+                out[n] = np.fft.irfftn(c[i], s=(nx, ny, nz), axes=(0, 1, 2))
     else:
         raise ValueError("unknown initial condition " + ic)
     if "bx" in out and ic in ("zero", "tg-yz", "tg3d"):
@@ -72,7 +79,7 @@ def real_fields(p, ic, seed=0, amplitude=1e-4, b0=0.0, k0=4.0):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("ic", choices=["zero", "tg-yz", "tg3d", "mhd-random", "mti-noise"])
+    ap.add_argument("ic", choices=["zero", "tg-yz", "tg3d", "mhd-random", "mti-noise", "hbi-noise"])
     ap.add_argument("--input-dir", default="./")
     ap.add_argument("--output-dir", default=None)
     ap.add_argument("--seed", type=int, default=0)

@@ -263,3 +263,57 @@ def test_mri_eigenmode_and_spectrum_file(tmp_path):
     assert len(user) == 2 + len(times)
     for ln in user[2:]:
         assert ln.split() == ["-1.00000000e+00", "0.00000000e+00"]  # kpartheta needs anisotropic conduction
+
+
+@pytest.mark.parametrize("problem, ic, t_end", [
+    ("problem-generic", ["tg-yz"], 0.05),
+    ("problem-hbi2d", ["hbi-noise", "--amplitude", "0.01", "--b0", "1e-5", "--seed", "98765"], 0.5),
+    ("problem-mti3d", ["tg-yz", "--b0", "1e-4"], 0.02),
+])
+def test_reference_problem_runs_unchanged(tmp_path, problem, ic, t_end):
+    """problems/{generic, hbi2d, mti3d}/spooky.cfg (kept under tests/golden/cfg)
+    run through the driver from the initial state of each problem's
+    initial-condition script (written by write_ic.py), with only t_final and
+    snapshot_step shortened to t_end; the final snapshot equals the FP64
+    oracle's state after the same steps to 1e-10, and the timevar, spectrum and
+    user files carry rows."""
+    import re
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc
+    write_ic_mod = importlib.import_module("spooky-git_amd.write_ic")
+    cfg = tmp_path / "cfg"
+    cfg.mkdir()
+    text = open(os.path.join(CFG, problem, "spooky.cfg")).read()
+    text, n1 = re.subn(r"t_final\s*=\s*[0-9.eE+-]+;", "t_final = %r;" % t_end, text)
+    text, n2 = re.subn(r"snapshot_step\s*=\s*[0-9.eE+-]+;", "snapshot_step = %r;" % t_end, text)
+    assert n1 == 1 and n2 == 1
+    (cfg / "spooky.cfg").write_text(text)
+    out = str(tmp_path / "run")
+    assert write_ic_mod.main(ic + ["--input-dir", str(cfg), "--output-dir", out]) == 0
+    p = json.loads(subprocess.run([SPOOKY, "--input-dir", str(cfg), "--output-dir", out, "--print-params"],
+                                  capture_output=True, text=True).stdout.strip().splitlines()[-1])
+    names = p["fields"]
+    shape = (p["nx"], p["ny"], p["nz"])
+    _, f0 = snapshot.read_snapshot(snapshot.snapshot_path(os.path.join(out, "data"), 0), names, shape)
+    run(str(cfg), out, "-r", "0")
+    snaps = read_all(out, names, shape)
+    hdr, f = snaps[-1]
+    assert hdr["t_save"] == pytest.approx(t_end, abs=1e-12)
+    q = {k: p[k] for k in orc.DEFAULTS if k in p}
+    sim = orc.Sim(**q)
+    sim.set_fields(np.stack([np.fft.rfftn(f0[k]) for k in names]))
+    while sim.time()["t"] < t_end - 1e-12:
+        sim.main_loop(1)
+    assert sim.time()["step"] == hdr["step"]
+    ref = {k: np.fft.irfftn(sim.fields[i], s=shape, axes=(0, 1, 2)) for i, k in enumerate(names)}
+    scale = max(np.max(np.abs(ref[k])) for k in names)
+    for k in names:
+        assert np.max(np.abs(f[k] - ref[k])) < 1e-10 * scale, (k, np.max(np.abs(f[k] - ref[k])))
+    data = os.path.join(out, "data")
+    tv = open(os.path.join(data, "timevar.spooky")).read().splitlines()
+    assert len(tv) >= 3 and all(len(ln.split()) == len(tv[1].split()) - 1 for ln in tv[2:])
+    sp = open(os.path.join(data, "spectrum.spooky")).read().splitlines()
+    assert sp[2].startswith("wavevector") and (len(sp) > 3) == (p["spectra_vars"] != [])
+    if p["user_timevar_vars"]:
+        assert len(open(os.path.join(data, "user-timevar.spooky")).read().splitlines()) == len(tv)
