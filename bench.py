@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--closure-spinup", type=float, default=1.0)
     ap.add_argument("--closure-iters", type=int, default=2000)
     ap.add_argument("--closure-dns", type=int, default=512)
+    ap.add_argument("--closure-samples", type=int, default=6)
+    ap.add_argument("--closure-sample-dt", type=float, default=0.05)
     ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure", "closure-kernels"],
                     default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
@@ -632,7 +634,8 @@ def main():
         return
     if a.mode == "closure":
         print(json.dumps(closure_leg(spk, ini, ctx, torch, N=a.closure_dns, t_spin=a.closure_spinup,
-                                     iters=a.closure_iters)))
+                                     iters=a.closure_iters, nsamples=a.closure_samples,
+                                     dt_sample=a.closure_sample_dt)))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
     if a.mode == "kernels":
