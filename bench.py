@@ -53,11 +53,11 @@ def parse():
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the Taylor-Green 64^3 and MTI legs")
     ap.add_argument("--ensemble-steps", type=int, default=20)
-    ap.add_argument("--closure-spinup", type=float, default=1.0)
-    ap.add_argument("--closure-iters", type=int, default=2000)
+    ap.add_argument("--closure-spinup", type=float, default=2.0)
+    ap.add_argument("--closure-iters", type=int, default=3000)
     ap.add_argument("--closure-dns", type=int, default=512)
-    ap.add_argument("--closure-samples", type=int, default=6)
-    ap.add_argument("--closure-sample-dt", type=float, default=0.05)
+    ap.add_argument("--closure-samples", type=int, default=12)
+    ap.add_argument("--closure-sample-dt", type=float, default=0.1)
     ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure", "closure-kernels"],
                     default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
