@@ -80,11 +80,24 @@ def unpack(params):
     return Ws, bs
 
 
-def reference(params, u, in_scale, tgt=None):
-    """numpy forward (and backward when tgt is given); u: [3][n][n][n]."""
+def reference(params, u, in_scale, tgt=None, exact=False):
+    """numpy forward (and backward when tgt is given); u: [3][n][n][n].  exact:
+    float64 throughout (no BF16 rounding), for checks of the restatement itself."""
+    global bf
+    rnd = bf
+    if exact:
+        bf = lambda x: np.asarray(x, dtype=np.float64)  # noqa: E731
+    try:
+        return _reference(params, u, in_scale, tgt)
+    finally:
+        bf = rnd
+
+
+def _reference(params, u, in_scale, tgt=None):
+    n = u.shape[-1]
     Ws, bs = unpack(params)
     Wb = [bf(w) for w in Ws]
-    x0 = np.zeros((N, N, N, 16))
+    x0 = np.zeros((n, n, n, 16))
     x0[..., :3] = bf(np.moveaxis(u.astype(np.float32) * np.float32(in_scale), 0, -1))
     acts, zs = [x0], []
     for l in range(3):
@@ -94,8 +107,8 @@ def reference(params, u, in_scale, tgt=None):
     out = conv(acts[-1], Wb[3])[..., :6] + bs[3][:6]
     if tgt is None:
         return out
-    P = N ** 3
-    d = np.zeros((N, N, N, 32))
+    P = n ** 3
+    d = np.zeros((n, n, n, 32))
     d[..., :6] = bf((2.0 / (6 * P)) * (out - tgt))
     grads = [None] * 4
     for l in (3, 2, 1, 0):
