@@ -147,6 +147,80 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
     }
 }
 
+// Output layer (32 -> 6 channels) on the 16-row matrix instruction
+// v_mfma_f32_16x16x32_bf16: D[co < 16][16 points] += W[tap][co][32 ci] X[point + tap][32 ci],
+// one instruction per tap and 16-point tile (the 6 stresses in rows 0..5), half
+// the work of a 32-row tile.  out[c][p] = D + b for c < 6.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr size_t OUT_LDS = (size_t)(NTAP * 16 * CH + HX * HY * HZ * CH) * sizeof(bf16);
+
+__global__ __launch_bounds__(CONV_T) void k_conv_out(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                                     const float *__restrict__ bias, float *__restrict__ out, int n) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    bf16 *Ws = reinterpret_cast<bf16 *>(smem);  // [27][16][32], chunk-swizzled by co
+    bf16 *Xs = Ws + NTAP * 16 * CH;             // [HX*HY*HZ][32], chunk-swizzled by halo point
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int l16 = lane & 15, q = lane >> 4;   // 16x16x32: lane holds row/column l16, K chunk q
+    const int xw = w & 3, yg = 2 * (w >> 2);
+
+    for (int i = tid; i < NTAP * 16 * 4; i += CONV_T) {
+        const int qq = i & 3, row = i >> 2;  // row = t * 16 + co
+        const int t = row >> 4, co = row & 15;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + ((size_t)t * CH + co) * CH + 8 * qq);
+        *reinterpret_cast<bf16x8 *>(Ws + row * CH + 8 * swz(qq, co, CH)) = v;
+    }
+    const int nbz = n / 32, nby = n / 4;
+    const int nb = (n / 4) * nby * nbz;
+    const size_t P = (size_t)n * n * n;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
+        __syncthreads();
+        for (int i = tid; i < HX * HY * HZ * 4; i += CONV_T) {
+            const int qq = i & 3, hp = i >> 2;
+            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
+            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
+            const size_t g = (((size_t)gx * n + gy) * n + gz) * CH + 8 * qq;
+            *reinterpret_cast<bf16x8 *>(Xs + hp * CH + 8 * swz(qq, hp, CH)) = *reinterpret_cast<const bf16x8 *>(X + g);
+        }
+        __syncthreads();
+        f32x4 acc[2][2];  // [column][16-point half]
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[c][h][r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NTAP; t++) {
+            const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(Ws + (t * 16 + l16) * CH + 8 * swz(q, l16, CH));
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int hp = ((xw + 1 + dx) * HY + (yg + c + 1 + dy)) * HZ + 16 * h + l16 + 1 + dz;
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * CH + 8 * swz(q, hp, CH));
+                    acc[c][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[c][h], 0, 0, 0);
+                }
+        }
+        // D[co][point]: lane holds point l16 of its half, co = 4 q + r
+        if (q < 2) {
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const size_t p = ((size_t)(x0 + xw) * n + (y0 + yg + c)) * n + z0 + 16 * h + l16;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int co = 4 * q + r;
+                        if (co < 6) out[(size_t)co * P + p] = acc[c][h][r] + bias[co];
+                    }
+                }
+        }
+    }
+}
+
 // weight gradient, per-block partial sums part[block][27][32 co][32 ci]
 constexpr int DZS = 512 + 8;                 // dZ^T row (one co): 512 brick points + pad (65 16-byte slots)
 constexpr int XPL = HX * HY * 48 + 8;        // X^T plane (one ci): z rows of 48 + pad (217 slots)
@@ -546,7 +620,7 @@ int set_lds_attributes() {
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv<16, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(16)));
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
-    SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
+    SPK_HIP(hipFuncSetAttribute((const void *)k_conv_out, hipFuncAttributeMaxDynamicSharedMemorySize, (int)OUT_LDS));
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv<32, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds(32)));
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv_dw<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)DW_LDS));
     SPK_HIP(hipFuncSetAttribute((const void *)k_conv_dw<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)DW_LDS));
@@ -585,7 +659,7 @@ int forward(spk_closure c, const void *u, bool train) {
         k_conv<32, 0><<<g, CONV_T, conv_lds(32), st>>>(c->A[0], c->Wb[1], b[1], c->A[1], nullptr, nullptr, nullptr, c->n);
         k_conv<32, 0><<<g, CONV_T, conv_lds(32), st>>>(c->A[1], c->Wb[2], b[2], c->A[2], nullptr, nullptr, nullptr, c->n);
     }
-    k_conv<32, 2><<<g, CONV_T, conv_lds(32), st>>>(c->A[2], c->Wb[3], b[3], nullptr, nullptr, nullptr, c->out, c->n);
+    k_conv_out<<<g, CONV_T, OUT_LDS, st>>>(c->A[2], c->Wb[3], b[3], c->out, c->n);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
