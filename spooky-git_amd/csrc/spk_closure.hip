@@ -312,35 +312,47 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
     }
 }
 
-// bias gradient partials: part[block][32] = sum over the block's points of dZ
+// bias gradient partials: part[block][32] = sum over the block's points of dZ.
+// Thread tid reads 8-channel chunk tid % 4 of its points (16-byte coalesced
+// loads); the wave sums lanes of equal chunk with shuffles, then the 4 waves.
 __global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ dZ, size_t P, float *__restrict__ part) {
-    __shared__ float sm[CH];
-    if (threadIdx.x < CH) sm[threadIdx.x] = 0.0f;
-    __syncthreads();
-    float acc[CH];
+    __shared__ float sm[4][CH];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    float acc[8];
 #pragma unroll
-    for (int c = 0; c < CH; c++) acc[c] = 0.0f;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+    for (int j = 0; j < 8; j++) acc[j] = 0.0f;
+    const size_t nchunk = P * 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + tid; i < nchunk; i += (size_t)gridDim.x * blockDim.x) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + i * 8);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
-#pragma unroll
-            for (int j = 0; j < 8; j++) acc[8 * q + j] += (float)v[j];
-        }
+        for (int j = 0; j < 8; j++) acc[j] += (float)v[j];
     }
 #pragma unroll
-    for (int c = 0; c < CH; c++) atomicAdd(&sm[c], acc[c]);
+    for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] += __shfl_xor(acc[j], o, 64);
+    if (lane < 4)
+#pragma unroll
+        for (int j = 0; j < 8; j++) sm[w][8 * lane + j] = acc[j];
     __syncthreads();
-    if (threadIdx.x < CH) part[(size_t)blockIdx.x * CH + threadIdx.x] = sm[threadIdx.x];
+    if (tid < CH) part[(size_t)blockIdx.x * CH + tid] = sm[0][tid] + sm[1][tid] + sm[2][tid] + sm[3][tid];
 }
 
-// grad[k] = sum_b part[b][k] for k < count
+__global__ void k_fill0(float *__restrict__ p, int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = 0.0f;
+}
+
+// grad[k] += sum over slice blockIdx.y of the parts: part[b][k] (grad zeroed before)
+constexpr int SUM_SLICES = 16;
 __global__ void k_sum_parts(const float *__restrict__ part, int nparts, int count, float *__restrict__ grad) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
+    const int per = (nparts + SUM_SLICES - 1) / SUM_SLICES;
+    const int b0 = blockIdx.y * per, b1 = b0 + per < nparts ? b0 + per : nparts;
     float s = 0.0f;
-    for (int b = 0; b < nparts; b++) s += part[(size_t)b * count + k];
-    grad[k] = s;
+    for (int b = b0; b < b1; b++) s += part[(size_t)b * count + k];
+    atomicAdd(&grad[k], s);
 }
 
 // padded real FP64 u [3][n][n][n + 2] -> BF16 [P][16] (channels 3..15 zero), times in_scale
@@ -673,14 +685,15 @@ int layer_grads(spk_closure c, int l, const bf16 *X, const bf16 *dz) {
     const int cnt = NTAP * CH * CH;
     // the [27][32][32] sums land in grd for cin = 32; for cin = 16 take ci < 16
     float *gw = c->grd + param_offset(l);
-    if (CIN_L[l] == 32) {
-        k_sum_parts<<<(cnt + 255) / 256, 256, 0, st>>>(c->part, (int)g, cnt, gw);
-    } else {
-        k_sum_parts<<<(cnt + 255) / 256, 256, 0, st>>>(c->part, (int)g, cnt, c->out);  // out is free here
+    const dim3 sg((cnt + 255) / 256, SUM_SLICES);
+    float *sum_to = CIN_L[l] == 32 ? gw : c->out;  // out is free here
+    k_fill0<<<(cnt + 255) / 256, 256, 0, st>>>(sum_to, cnt);
+    k_sum_parts<<<sg, 256, 0, st>>>(c->part, (int)g, cnt, sum_to);
+    if (CIN_L[l] != 32)
         SPK_HIP(hipMemcpy2DAsync(gw, 16 * 4, c->out, 32 * 4, 16 * 4, NTAP * CH, hipMemcpyDeviceToDevice, st));
-    }
     k_bias_grad<<<NBLK, 256, 0, st>>>(dz, c->P, c->bpart);
-    k_sum_parts<<<1, CH, 0, st>>>(c->bpart, NBLK, CH, gw + w_count(l));
+    k_fill0<<<1, CH, 0, st>>>(gw + w_count(l), CH);
+    k_sum_parts<<<dim3(1, SUM_SLICES), CH, 0, st>>>(c->bpart, NBLK, CH, gw + w_count(l));
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
