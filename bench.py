@@ -607,8 +607,9 @@ def closure_kernels(spk, ctx, torch, n=128, reps=20):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: every box here has a single MI355X and this build has no "
-              "multi-GPU path (--gpus %d refused)" % a.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only: the solver has no multi-GPU path; its sharding is a design "
+              "(slab decomposition with one all-to-all per 3D transform) (--gpus %d refused)" % a.gpus,
+              file=sys.stderr)
         sys.exit(2)
     import torch
 

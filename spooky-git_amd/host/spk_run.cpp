@@ -204,7 +204,7 @@ bool read_run_params(const Config &c, const CliArgs &cli, RunParams *r, std::str
     p.sts_mask = 0;
     if (p.supertimestepping) {
         if (r->sts_algorithm != "rkl2") {
-            *err = "sts_algorithm \"" + r->sts_algorithm + "\" is not available: this build runs \"rkl2\"";
+            *err = "sts_algorithm \"" + r->sts_algorithm + "\" is not available: this solver runs \"rkl2\"";
             return false;
         }
         const std::vector<std::string> names = field_names(p);

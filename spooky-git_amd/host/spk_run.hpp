@@ -51,7 +51,7 @@ struct RunParams {
 };
 
 // Fills r from cfg and the command line overrides; returns false with a
-// message for settings this build cannot run (an sts algorithm other than rkl2).
+// message for settings this solver cannot run (an sts algorithm other than rkl2).
 bool read_run_params(const Config &cfg, const CliArgs &cli, RunParams *r, std::string *err);
 
 // checkParameters (parameters.cu:510-566): 1 when consistent; messages go to
