@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import pyoracle as orc
-from physics_cases import CASES, FUSED_CASES, initial_state, relerr
+from physics_cases import CASES, FUSED_CASES, ini, initial_state, relerr
 from spkmod import spk
 
 pytestmark = pytest.mark.gpu
@@ -173,3 +173,26 @@ def test_fused_path_matches_unfused_and_oracle(ctx, name, prec):
     assert relerr(sol2.get_fields(), a) < (1e-13 if prec == "f64" else 1e-6)
     t = sol.kernel_timings(2)
     assert t["fwdx_epilogue"] is not None and t["fwdx_epilogue"] > 0
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_fused_matches_unfused_at_256(ctx, prec):
+    """At 256^3 MHD (2064 column tiles of 16 per field group) the fused kernel
+    gives the unfused pipeline's state after a two-step call and a one-step call
+    (the second call starts from a fresh W)."""
+    n = 256
+    p = dict(nx=n, ny=n, nz=n, incompressible=1, mhd=1, lx=2 * np.pi, ly=2 * np.pi, lz=2 * np.pi, nu=1e-3,
+             nu_m=1e-3, cfl=0.5, t_final=1e9, toutput_flow=1e9)
+    shape = (n, n, n)
+    u0 = np.concatenate([ini.random_solenoidal(shape, seed=11), ini.random_solenoidal(shape, seed=12, rms=0.5)])
+    out = {}
+    for mode in (0, 2):
+        sol = spk.Solver(ctx, prec, **p)
+        sol.set_fused(mode)
+        sol.set_fields(u0)
+        sol.main_loop(2)
+        sol.main_loop(1)
+        out[mode] = sol.get_fields()
+        sol.destroy()
+    tol = 1e-12 if prec == "f64" else 2e-5
+    assert relerr(out[2], out[0]) < tol, relerr(out[2], out[0])
