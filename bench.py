@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="oracle steps of the CPU baseline, also compared with the GPU path (accuracy_vs_oracle)")
     ap.add_argument("--precisions", default="f64,f32")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the FP32-bulk vs FP64 accuracy run (0: skip)")
@@ -148,7 +149,9 @@ def kernel_table(spk, ctx, state, n, prec, fused, reps=10):
     nin, nout, nf = 6, 8, 6
     algo = {"inv_x": nin * (s64 + sb), "inv_y": nin * 2 * sb, "zmid": (nin + nout) * sb, "fwd_y": nout * 2 * sb,
             "fwd_x": nout * 2 * sb, "epilogue": nout * sb + 4 * nf * s64, "clean_div": 2 * nf * s64,
-            "fwdx_epilogue": nout * sb + 4 * nf * s64 + nin * sb}
+            # stage 2: products in, U0 and the increment register in, U2 and the new increment out
+            # (the register holds the bulk precision), next inverse x pass out
+            "fwdx_epilogue": nout * sb + 2 * nf * s64 + 2 * nf * sb + nin * sb}
     if fused:
         # fused pipeline: forward x + epilogue + projection + next inverse x in one kernel
         per_step = {"inv_y": 3, "zmid": 3, "fwd_y": 3, "fwdx_epilogue": 3}
@@ -182,6 +185,8 @@ def copy_gbs(torch, nbytes):
 
 
 def cpu_baseline(n, state, steps):
+    """The FP64 oracle timed over `steps` steps of the same workload; returns the
+    baseline record and the oracle's state and energy after those steps."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as orc
 
@@ -191,9 +196,33 @@ def cpu_baseline(n, state, steps):
     sim.main_loop(steps)
     dt = time.perf_counter() - t0
     cores = orc.lib().orc_num_threads()
-    return dict(value=steps / dt, unit="steps/s", cores=cores, kind="port",
-                sample="%d MHD RK3 step(s) at %d^3, FP64 oracle (C, OpenMP), same initial state" % (steps, n),
-                seconds=dt)
+    rec = dict(value=steps / dt, unit="steps/s", cores=cores, kind="port",
+               sample="%d MHD RK3 step(s) at %d^3, FP64 oracle (C, OpenMP), same initial state" % (steps, n),
+               seconds=dt)
+    out = dict(fields=sim.fields.copy(), energy=sum(sim.energy(f) for f in range(6)), t=sim.time()["t"])
+    return rec, out
+
+
+def accuracy_vs_oracle(spk, ctx, state, n, steps, orc_out):
+    """The GPU path (FP64 and FP32 bulk, the back end the benchmark runs) against
+    the FP64 oracle after the same `steps` steps from the same state: full-state
+    relative L2 and relative total-energy error."""
+    import numpy as np
+    res = dict(steps=steps)
+    ref = orc_out["fields"]
+    nref = float(np.linalg.norm(ref.ravel()))
+    for prec in ("f64", "f32"):
+        sol = spk.Solver(ctx, prec, **mhd_params(n))
+        sol.set_fields(state)
+        sol.main_loop(steps)
+        got = sol.get_fields()
+        e = sum(sol.energy(f) for f in range(6))
+        res[prec] = dict(rel_l2_state=float(np.linalg.norm((got - ref).ravel())) / nref,
+                         rel_energy_err=abs(e - orc_out["energy"]) / abs(orc_out["energy"]),
+                         rel_time_err=abs(sol.time()["t"] - orc_out["t"]) / orc_out["t"])
+        sol.destroy()
+        del got
+    return res
 
 
 def rkl2_stages(dt, dt_sts):
@@ -692,9 +721,14 @@ def main():
     ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps) if a.extra and a.ensemble_steps > 0 else None
 
     cpu = None
+    acc_orc = None
     if not a.no_cpu:
-        cpu = cpu_baseline(n, state, a.cpu_steps)
-        print("# cpu oracle: %.3f s/step on %d threads" % (1.0 / cpu["value"], cpu["cores"]), file=sys.stderr)
+        cpu, orc_out = cpu_baseline(n, state, a.cpu_steps)
+        print("# cpu oracle: %.3f s/step on %d threads" % (1.0 / cpu["value"], cpu["cores"]), file=sys.stderr,
+              flush=True)
+        acc_orc = accuracy_vs_oracle(spk, ctx, state, n, a.cpu_steps, orc_out)
+        del orc_out
+        print("# gpu vs oracle after %d steps: %s" % (a.cpu_steps, json.dumps(acc_orc)), file=sys.stderr, flush=True)
 
     f64 = res.get("f64") or next(iter(res.values()))
     line = {
@@ -729,6 +763,7 @@ def main():
         "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
                         for k, v in kern.items()},
         "roofline": roofline,
+        "accuracy_vs_oracle": acc_orc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
