@@ -208,6 +208,67 @@ int spk_user_timevar(spk_solver s, const char *name, double *out);
  * exp(-i ky Lx S fmod(t, 2 Ly / (S Lx)) / 2).  Synchronizes to read t. */
 int spk_snapshot_real(spk_solver s);
 
+/* ---- the same operations on caller-owned arrays ----
+ * The reference's methods take the field arrays as arguments and its host
+ * classes allocate every device array (Supervisor::spookyGpuAlloc,
+ * supervisor.hpp:74-78).  These entries run the solver's kernels on arrays the
+ * caller owns (allocated with spk_malloc or any hipMalloc on the context's
+ * device): complex_fields = d_all_fields, [nf][NX][NY][NZ/2+1] complex FP64;
+ * real_buffer = d_all_buffer_r, [nf][NX][NY][2(NZ/2+1)] FP64.  The solver handle
+ * supplies the parameters, the grid and the library's transform buffers.
+ * Complex arrays must be 16-byte aligned.  A real_buffer documented as not read
+ * may be NULL. */
+/* TimeStepping::compute_dt(complexFields, realBuffer) (timestepping.hpp:39,
+ * compute_dt.cu:15-309): dt, dt_hyp, dt_par, dt_sts on the device; when
+ * real_buffer is not NULL it receives the real fields, as the reference fills it */
+int spk_compute_dt_ptr(spk_solver s, const void *complex_fields, double *real_buffer);
+/* TimeStepping::compute_dfield(complex_Fields, real_Buffer, complex_dFields)
+ * (timestepping.hpp:40, compute_dfield.cu:11-90); real_buffer is not read */
+int spk_compute_dfield_ptr(spk_solver s, const void *complex_fields, double *real_buffer, void *complex_dfields);
+/* TimeStepping::RungeKutta3(complex_Fields, real_Buffer) (timestepping.hpp:42,
+ * hydro_mhd_advance.cu:82-189) with the dt of the last compute_dt; scratch =
+ * d_all_scrtimestep (timestepping.cu:36), nf fields, or NULL for the solver's
+ * own; real_buffer: scratch of the shearing-box remap only.  RK3 stage 1 reuses
+ * the products of a compute_dt on the same complex_fields, as the reference
+ * reuses the real buffer compute_dt filled. */
+int spk_rk3_ptr(spk_solver s, void *complex_fields, double *real_buffer, void *scratch);
+/* TimeStepping::HydroMHDAdvance (hydro_mhd_advance.cu:18-75) on caller arrays */
+int spk_hydro_mhd_advance_ptr(spk_solver s, void *complex_fields, double *real_buffer, void *scratch);
+/* Fields::CleanFieldDivergence (symmetries.cu:114-130) on caller fields */
+int spk_clean_divergence_ptr(spk_solver s, void *complex_fields);
+/* CleanDivergence of one 3-component vector field (cuda_kernels_generic.cu:288-313) */
+int spk_project(spk_solver s, void *complex_vec);
+/* Supervisor::Complex2RealFields(ComplexField_in, RealField_out, num_fields)
+ * (supervisor.hpp:72, supervisor.cu:261-272); the input is not modified */
+int spk_complex2real_ptr(spk_solver s, const void *complex_in, double *real_out, int num_fields);
+/* RKLegendre::compute_cycle(complex_Fields, real_Buffer) (rkl.hpp:42, RKL2 cycle
+ * rkl.cu:437-591); synchronizes to size the cycle; real_buffer is not read */
+int spk_rkl2_cycle_ptr(spk_solver s, void *complex_fields, double *real_buffer);
+/* One stage of that cycle (rkl.cu:496-585): ParabolicTermsSTS of the sts
+ * variables (physics.cu:233-330, with AnisotropicHeatFlux for theta) and the
+ * rkl2_stage update (cuda_kernels_generic.cu:160-172).  coef = mu_j, nu_j,
+ * mu~_j, gamma~_j of the recurrence (rkl.cu:466-585; stage 1 uses mu~_1 only);
+ * uc0, uc1, du0: the RKLegendre arrays d_all_Uc0, d_all_Uc1, d_all_dU0
+ * (rkl.cu:125-153), num_sts_vars fields each, caller-owned. */
+int spk_rkl2_stage(spk_solver s, int stage, const double *coef, double dt, void *complex_fields, void *uc0,
+                   void *uc1, void *du0);
+/* Physics::AnisotropicHeatFlux(complex_Fields, real_Buffer, complex_dTheta)
+ * (physics.hpp:27, anisotropic_conduction.cu:10-62): complex_dtheta += i mask
+ * k . FFT[chi b (b . grad theta + OmegaT2 b_s) / B^2]; real_buffer is not read */
+int spk_aniso_flux(spk_solver s, const void *complex_fields, double *real_buffer, void *complex_dtheta);
+/* max |x| over the physical points of one padded real field into a device
+ * double: cublasIdamax + the read-back of the reference (compute_dt.cu:90-106)
+ * without the host round trip; asynchronous */
+int spk_maxabs(spk_solver s, const double *real_field, double *out_dev);
+/* computeEnergy(data_type *vcomplex) (spooky_output_functions.cu:23-55); synchronizes */
+int spk_energy_ptr(spk_solver s, const void *complex_field, double *out);
+/* computeEnstrophy(data_type *complex_vecField) (spooky_output_functions.cu:57-86) */
+int spk_enstrophy_ptr(spk_solver s, const void *complex_vec, double *out);
+/* twoFieldCorrelation(scalar_type *v1, scalar_type *v2) (spooky_output_functions.cu:188-235) */
+int spk_correlation_ptr(spk_solver s, const double *real_a, const double *real_b, double *out);
+/* computeSpectrum1d(v1, v2, ...) (inputoutput.hpp:30, spooky_output_spectrum.cu:424-444) */
+int spk_spectrum_ptr(spk_solver s, const void *complex_a, const void *complex_b, double *out, int cap, int *nbins);
+
 /* ---- batched ensemble: many independent boxes advanced together ----
  * The reference runs one box per process (Supervisor, supervisor.cu:20-213);
  * an ensemble holds nboxes solvers (box i: params[i], any grid and physics the

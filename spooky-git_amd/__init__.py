@@ -133,6 +133,21 @@ SIGNATURES = {
     "spk_closure_apply": [_VP, _VP],
     "spk_sgs_stress": [_VP, _VP, _I, _VP, _VP, _I],
     "spk_solver_set_forcing": [_VP, _I, _VP, _VP],
+    "spk_compute_dt_ptr": [_VP, _VP, _VP],
+    "spk_compute_dfield_ptr": [_VP, _VP, _VP, _VP],
+    "spk_rk3_ptr": [_VP, _VP, _VP, _VP],
+    "spk_hydro_mhd_advance_ptr": [_VP, _VP, _VP, _VP],
+    "spk_clean_divergence_ptr": [_VP, _VP],
+    "spk_project": [_VP, _VP],
+    "spk_complex2real_ptr": [_VP, _VP, _VP, _I],
+    "spk_rkl2_cycle_ptr": [_VP, _VP, _VP],
+    "spk_rkl2_stage": [_VP, _I, _VP, _D, _VP, _VP, _VP, _VP],
+    "spk_aniso_flux": [_VP, _VP, _VP, _VP],
+    "spk_maxabs": [_VP, _VP, _VP],
+    "spk_energy_ptr": [_VP, _VP, _VP],
+    "spk_enstrophy_ptr": [_VP, _VP, _VP],
+    "spk_correlation_ptr": [_VP, _VP, _VP, _VP],
+    "spk_spectrum_ptr": [_VP, _VP, _VP, _VP, _I, _VP],
 }
 
 

@@ -489,7 +489,8 @@ __global__ __launch_bounds__(256) void k_diag_bvec(const double *__restrict__ B,
 
 // DivergenceMask (cuda_kernels_generic.cu:315-334): i mask (k . X), written
 // (acc 0, flag 0 there) or added to Z (acc 1, flag 1 there)
-__global__ __launch_bounds__(256) void k_diag_div(const cpx<double> *__restrict__ X, cpx<double> *__restrict__ Z,
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_div(const cpx<T> *__restrict__ X, cpx<double> *__restrict__ Z,
                                                   long long ntc, KGeom kg, int acc) {
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < ntc;
          idx += (long long)gridDim.x * blockDim.x) {
@@ -498,7 +499,8 @@ __global__ __launch_bounds__(256) void k_diag_div(const cpx<double> *__restrict_
         const int j = (int)(rowi % kg.ny), i = (int)(rowi / kg.ny);
         double kx, ky, kz, m;
         kvec_at(kg, i, j, k, kx, ky, kz, m);
-        const cpx<double> d = kdot3(kx, X[idx], ky, X[ntc + idx], kz, X[2 * ntc + idx]);
+        const cpx<double> d = kdot3(kx, cconv<double>(X[idx]), ky, cconv<double>(X[ntc + idx]), kz,
+                                    cconv<double>(X[2 * ntc + idx]));
         const cpx<double> z0 = acc ? Z[idx] : cpx<double>{0.0, 0.0};
         Z[idx] = cpx<double>{z0.x - m * d.y, z0.y + m * d.x};
     }
@@ -690,6 +692,7 @@ struct spk_solver_s {
     int step;
     double tremap;
     bool loop_started, products_valid;
+    const void *products_src;  // the state whose RHS products P holds when products_valid
     int fused_mode;      // caller's switch for k_fwdx_epilogue: 0 never, 1 autotuned, 2 always
     bool fused_faster;   // tile autotune: the fused back end beat the unfused one on this grid
     int fused_ef;        // points per thread of its x line
@@ -815,6 +818,10 @@ int launch_epilogue(spk_solver s, const EpiConst &c, int mode) {
     return SPK_OK;
 }
 
+// Wray's low-storage RK3 coefficients (hydro_mhd_advance.cu:79-80)
+static const double gammaRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double xiRK[2] = {-17.0 / 60.0, -5.0 / 12.0};
+
 // ---- pipeline pieces ----
 
 // inverse transforms of the RHS inputs and the fused z kernel: products in P
@@ -893,6 +900,10 @@ int rhs_back_fused(spk_solver s, int mode, double gam, double xi, int project, i
     fc.project = project;
     fc.inv = inv;
     fc.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+    // the increment register's coefficients (FusedConst): stage 2 forms U1 = U0 + g1 R0
+    // and S1 = U0 + (g1 + x1) R0, stage 3 forms S2 = U2 + x2 R1
+    fc.cu = mode == 2 ? gammaRK[0] : 0.0;
+    fc.cs = mode == 2 ? gammaRK[0] + xiRK[0] : (mode == 3 ? xiRK[1] : 0.0);
     const void *tw = (const char *)s->twb.d + (size_t)s->twb.off_x * (s->bulk == SPK_F64 ? 16 : 8);
     return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, mode, s->fused_ef, s->P, s->U, s->S, s->W, fc, tw,
                                 s->ts, fused_groups(s));
@@ -961,12 +972,11 @@ int compute_dt(spk_solver s) {
     if (s->p.incompressible && !s->p.heat_equation) {
         SPK_TRY(rhs_front(s, s->tremap * s->p.shear, true));
         s->products_valid = true;
+        s->products_src = s->U;
     }
     return dt_kernel(s);
 }
 
-static const double gammaRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-static const double xiRK[2] = {-17.0 / 60.0, -5.0 / 12.0};
 
 int shift_time_host(spk_solver s, double t) {
     const spk_params &p = s->p;
@@ -998,7 +1008,7 @@ int rk3(spk_solver s) {
     }
     const double ks0 = s->tremap * p.shear;
     // stage 1
-    if (!s->products_valid) SPK_TRY(rhs_front(s, ks0, false));
+    if (!s->products_valid || s->products_src != s->U) SPK_TRY(rhs_front(s, ks0, false));
     s->products_valid = false;
     SPK_TRY(rhs_back(s, ks0, 1, gammaRK[0], xiRK[0]));
     // stage 2
@@ -2199,4 +2209,248 @@ extern "C" int spk_solver_pipeline(spk_solver s, int *fused, int *points) {
     if (fused) *fused = fused_ok(s) ? 1 : 0;
     if (points) *points = s->fused_ef;
     return SPK_OK;
+}
+
+// ==========================================================================
+// C ABI on caller-owned buffers
+// ==========================================================================
+// The reference's Physics, TimeStepping and RKLegendre methods take the field
+// arrays as arguments (physics.hpp:18-42, timestepping.hpp:39-43, rkl.hpp:42) and
+// its host classes own every device array (Supervisor::spookyGpuAlloc,
+// supervisor.hpp:74-78).  The *_ptr entries run the solver's kernels on arrays the
+// caller owns: for the duration of the call the solver's own state arrays are
+// replaced by the caller's, and every kernel is enqueued with the caller's
+// addresses.  The solver keeps only its transform buffers, twiddles and
+// reduction partials.
+
+__global__ __launch_bounds__(256) void k_maxabs(const double *__restrict__ r, long long nrows, int nz, int nzp,
+                                                unsigned long long *__restrict__ out) {
+    __shared__ double sm[4];
+    double m = 0.0;
+    const long long n = nrows * nz;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long row = p / nz;
+        m = fmax(m, fabs(r[row * nzp + (p - row * nz)]));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sm[w]);
+        // non-negative doubles order as their bit patterns
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p) {
+    if (threadIdx.x == 0) *p = 0ull;
+}
+
+namespace {
+
+struct Borrow {
+    spk_solver s;
+    cpx<double> *U, *S, *dU, *uc0, *uc1, *du0;
+    double *rb;
+    Borrow(spk_solver s_, const void *u, void *scr, void *du, double *r)
+        : s(s_), U(s_->U), S(s_->S), dU(s_->dU), uc0(s_->rUc0), uc1(s_->rUc1), du0(s_->rdU0), rb(s_->realbuf) {
+        if (u) s->U = static_cast<cpx<double> *>(const_cast<void *>(u));
+        if (scr) s->S = static_cast<cpx<double> *>(scr);
+        if (du) s->dU = static_cast<cpx<double> *>(du);
+        if (r) s->realbuf = r;
+    }
+    ~Borrow() {
+        s->U = U; s->S = S; s->dU = dU; s->realbuf = rb;
+        s->rUc0 = uc0; s->rUc1 = uc1; s->rdU0 = du0;
+    }
+};
+
+bool aligned16(const void *p) { return p != nullptr && ((uintptr_t)p & 15u) == 0; }
+
+#define SPK_CHECK_PTR(p, what) \
+    SPK_REQUIRE(aligned16(p), SPK_ERR_ARG, "%s: %s is null or not 16-byte aligned", __func__, what)
+
+int complex2real_into(spk_solver s, const void *in, double *out, int nf) {
+    const Grid3 &g = s->g;
+    PassArgs a{in, out, nf, g.ntc, g.ntc, 1.0 / ((double)g.nx * g.ny * g.nz)};
+    SPK_TRY(launch_pass_x(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, a));
+    PassArgs b{out, out, nf, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, b));
+    return launch_pass_z_c2r(s->ctx, g, s->tw64, SPK_F64, SPK_F64, b);
+}
+
+// transform buffers of the anisotropic flux, made at the first call that needs them
+int need_flux_buffers(spk_solver s) {
+    if (s->W2 && s->P2) return SPK_OK;
+    const size_t fb = (size_t)s->g.ntc * esize(s->bulk);
+    if (!s->W2) SPK_TRY(dalloc(&s->W2, fb * 6));
+    if (!s->P2) SPK_TRY(dalloc(&s->P2, fb * 3));
+    return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" int spk_compute_dt_ptr(spk_solver s, const void *complex_fields, double *real_buffer) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    Borrow b(s, complex_fields, nullptr, nullptr, real_buffer);
+    SPK_TRY(compute_dt(s));
+    if (real_buffer) SPK_TRY(complex2real_into(s, complex_fields, real_buffer, s->nf));
+    return SPK_OK;
+}
+
+extern "C" int spk_compute_dfield_ptr(spk_solver s, const void *complex_fields, double *real_buffer,
+                                      void *complex_dfields) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    SPK_CHECK_PTR(complex_dfields, "complex_dfields");
+    (void)real_buffer;  // the products are formed from complex_fields
+    Borrow b(s, complex_fields, nullptr, complex_dfields, nullptr);
+    const double ks = s->tremap * s->p.shear;
+    SPK_TRY(rhs_front(s, ks, false));
+    s->products_valid = false;
+    return rhs_back(s, ks, 0, 0.0, 0.0);
+}
+
+extern "C" int spk_rk3_ptr(spk_solver s, void *complex_fields, double *real_buffer, void *scratch) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    SPK_REQUIRE(scratch == nullptr || aligned16(scratch), SPK_ERR_ARG, "spk_rk3_ptr: scratch not 16-byte aligned");
+    Borrow b(s, complex_fields, scratch, nullptr, real_buffer);
+    return rk3(s);
+}
+
+extern "C" int spk_hydro_mhd_advance_ptr(spk_solver s, void *complex_fields, double *real_buffer, void *scratch) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    SPK_REQUIRE(scratch == nullptr || aligned16(scratch), SPK_ERR_ARG,
+                "spk_hydro_mhd_advance_ptr: scratch not 16-byte aligned");
+    Borrow b(s, complex_fields, scratch, nullptr, real_buffer);
+    return hydro_mhd_advance(s);
+}
+
+extern "C" int spk_clean_divergence_ptr(spk_solver s, void *complex_fields) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    Borrow b(s, complex_fields, nullptr, nullptr, nullptr);
+    return clean_divergence(s);
+}
+
+extern "C" int spk_project(spk_solver s, void *complex_vec) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_vec, "complex_vec");
+    k_clean_div<<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>(static_cast<cpx<double> *>(complex_vec), s->g.ntc,
+                                                            kgeom(s, s->tremap * s->p.shear), 0, -1);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+extern "C" int spk_complex2real_ptr(spk_solver s, const void *complex_in, double *real_out, int num_fields) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_in, "complex_in");
+    SPK_CHECK_PTR(real_out, "real_out");
+    SPK_REQUIRE(num_fields >= 1, SPK_ERR_ARG, "spk_complex2real_ptr: num_fields %d", num_fields);
+    return complex2real_into(s, complex_in, real_out, num_fields);
+}
+
+extern "C" int spk_rkl2_cycle_ptr(spk_solver s, void *complex_fields, double *real_buffer) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    Borrow b(s, complex_fields, nullptr, nullptr, real_buffer);
+    return rkl2_cycle(s);
+}
+
+extern "C" int spk_rkl2_stage(spk_solver s, int stage, const double *coef, double dt, void *complex_fields,
+                              void *uc0, void *uc1, void *du0) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(s->nsts > 0, SPK_ERR_STATE, "spk_rkl2_stage: no sts variables in this run");
+    SPK_REQUIRE(stage >= 1 && coef, SPK_ERR_ARG, "spk_rkl2_stage: bad stage %d or null coef", stage);
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    SPK_CHECK_PTR(uc0, "uc0");
+    SPK_CHECK_PTR(uc1, "uc1");
+    SPK_CHECK_PTR(du0, "du0");
+    Borrow b(s, complex_fields, nullptr, nullptr, nullptr);
+    s->rUc0 = static_cast<cpx<double> *>(uc0);
+    s->rUc1 = static_cast<cpx<double> *>(uc1);
+    s->rdU0 = static_cast<cpx<double> *>(du0);
+    if (s->rkl_aniso) SPK_TRY(rkl2_front(s));
+    if (s->bulk == SPK_F64) return rkl2_stage_launch<double>(s, stage, coef[0], coef[1], coef[2] * dt, coef[3] * dt);
+    return rkl2_stage_launch<float>(s, stage, coef[0], coef[1], coef[2] * dt, coef[3] * dt);
+}
+
+extern "C" int spk_aniso_flux(spk_solver s, const void *complex_fields, double *real_buffer, void *complex_dtheta) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(s->p.anisotropic_diffusion, SPK_ERR_STATE, "spk_aniso_flux: anisotropic_diffusion is off");
+    SPK_CHECK_PTR(complex_fields, "complex_fields");
+    SPK_CHECK_PTR(complex_dtheta, "complex_dtheta");
+    (void)real_buffer;  // B and grad theta are transformed from complex_fields
+    SPK_TRY(need_flux_buffers(s));
+    Borrow b(s, complex_fields, nullptr, nullptr, nullptr);
+    SPK_TRY(rkl2_front(s));
+    KGeom kg = kgeom(s, s->tremap * s->p.shear);
+    cpx<double> *z = static_cast<cpx<double> *>(complex_dtheta);
+    const unsigned gb = red_grid(s->g.ntc);
+    if (s->bulk == SPK_F64)
+        k_diag_div<<<gb, 256, 0, s->ctx->stream>>>((const cpx<double> *)s->P2, z, s->g.ntc, kg, 1);
+    else
+        k_diag_div<<<gb, 256, 0, s->ctx->stream>>>((const cpx<float> *)s->P2, z, s->g.ntc, kg, 1);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+extern "C" int spk_maxabs(spk_solver s, const double *real_field, double *out_dev) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(real_field && out_dev && ((uintptr_t)out_dev & 7u) == 0, SPK_ERR_ARG, "spk_maxabs: bad pointers");
+    unsigned long long *o = reinterpret_cast<unsigned long long *>(out_dev);
+    k_zero_u64<<<1, 64, 0, s->ctx->stream>>>(o);
+    const long long rows = (long long)s->g.nx * s->g.ny;
+    k_maxabs<<<red_grid(rows * s->g.nz), 256, 0, s->ctx->stream>>>(real_field, rows, s->g.nz, 2 * s->g.nzh, o);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+extern "C" int spk_energy_ptr(spk_solver s, const void *complex_field, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_field, "complex_field");
+    SPK_REQUIRE(out, SPK_ERR_ARG, "spk_energy_ptr: null out");
+    const unsigned nb = red_grid(s->g.ntc);
+    k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(static_cast<const cpx<double> *>(complex_field), s->g.ntc, s->kg0, 0,
+                                               s->part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)s->g.nx * s->g.ny * s->g.nz;
+    return reduce_final(s, (int)nb, 1.0 / (nt * nt), out);
+}
+
+extern "C" int spk_enstrophy_ptr(spk_solver s, const void *complex_vec, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_vec, "complex_vec");
+    SPK_REQUIRE(out, SPK_ERR_ARG, "spk_enstrophy_ptr: null out");
+    const unsigned nb = red_grid(s->g.ntc);
+    k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(static_cast<const cpx<double> *>(complex_vec), s->g.ntc,
+                                               kgeom(s, s->tremap * s->p.shear), 1, s->part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)s->g.nx * s->g.ny * s->g.nz;
+    return reduce_final(s, (int)nb, 1.0 / (nt * nt), out);
+}
+
+extern "C" int spk_correlation_ptr(spk_solver s, const double *real_a, const double *real_b, double *out) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(real_a && real_b && out, SPK_ERR_ARG, "spk_correlation_ptr: null argument");
+    const long long rows = (long long)s->g.nx * s->g.ny;
+    const unsigned nb = red_grid(rows * s->g.nz);
+    k_real_dot<<<nb, 256, 0, s->ctx->stream>>>(real_a, real_b, rows, s->g.nz, 2 * s->g.nzh, s->part);
+    SPK_LAUNCH_CHECK();
+    return reduce_final(s, (int)nb, 1.0 / ((double)s->g.nx * s->g.ny * s->g.nz), out);
+}
+
+extern "C" int spk_spectrum_ptr(spk_solver s, const void *complex_a, const void *complex_b, double *out, int cap,
+                                int *nbins) {
+    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_PTR(complex_a, "complex_a");
+    SPK_CHECK_PTR(complex_b, "complex_b");
+    SPK_REQUIRE(out && cap >= 0, SPK_ERR_ARG, "spk_spectrum_ptr: bad output");
+    if (nbins) *nbins = s->nbins;
+    return spectrum_of(s, static_cast<const cpx<double> *>(complex_a), static_cast<const cpx<double> *>(complex_b),
+                       out, cap);
 }
