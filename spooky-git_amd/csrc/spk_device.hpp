@@ -349,6 +349,12 @@ __host__ __device__ constexpr int strided_tcols(int n) {
     // in FP64), wider HBM segments for the strided line accesses
     return strided_tpl(n) >= 512 ? 1 : 512 / strided_tpl(n);
 }
+// FP32 lines take twice the columns (1024-thread blocks at n = 256), so a line
+// point is again a 512-byte contiguous segment
+template <class T>
+__host__ __device__ constexpr int strided_tcols_t(int n) {
+    return sizeof(T) == 4 && strided_tpl(n) <= 16 ? 2 * strided_tcols(n) : strided_tcols(n);
+}
 template <int N>
 __host__ __device__ constexpr bool line_needs_lds() {
     return strided_e(N) != N;

@@ -26,13 +26,16 @@ namespace spk {
 
 // Line FFT over columns: element (f, o, n, c) at f*fstride + o*ostride + n*lstride + c.
 // AXIS (0 = x, 1 = y) only names the instantiation so profiles tell the passes apart.
-// TC columns per block; the y pass of a quasi-2D grid (NZ = 2: two columns of
-// adjacent complex values per x) uses TC = 2, so that every lane has a column
-// and a wave reads 32 consecutive lines' two values as one contiguous run.
+// TC columns per block, taken from the flattened (o, c) index, so a tile may
+// span two planes: the y pass of a 256^3 grid has 129 columns per x-plane, and
+// tiles per plane would leave one block in five with a single live column.
+// The y pass of a quasi-2D grid (NZ = 2: two columns of adjacent complex values
+// per x) uses TC = 2 with one plane per block, so that every lane has a column and
+// a wave reads 32 consecutive lines' two values as one contiguous run.
 template <class T, int N, int DIR, int AXIS, int TC = strided_tcols(N)>
 __global__ __launch_bounds__(TC * strided_tpl(N)) void k_pass_strided(
     const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
-    int ntiles, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale,
+    int nouter, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale,
     const int *__restrict__ gate) {
     if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
@@ -43,11 +46,12 @@ __global__ __launch_bounds__(TC * strided_tpl(N)) void k_pass_strided(
     const int tid = threadIdx.x;
     const int c = tid % TC, t = tid / TC;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = b % ntiles, o = b / ntiles;
-    const long long col = (long long)tile * TC + c;
-    const bool active = col < ncols;
-    const long long base_in = (long long)blockIdx.y * fs_in + (long long)o * ostride + col;
-    const long long base_out = (long long)blockIdx.y * fs_out + (long long)o * ostride + col;
+    const long long flat = (long long)b * TC + c;
+    const long long o = flat / ncols;
+    const long long col = flat - o * ncols;
+    const bool active = o < nouter;
+    const long long base_in = (long long)blockIdx.y * fs_in + o * ostride + col;
+    const long long base_out = (long long)blockIdx.y * fs_out + o * ostride + col;
 
     cpx<T> v[E];
 #pragma unroll
@@ -178,20 +182,20 @@ static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride
                           int nouter, const cpx<T> *twp, const PassArgs &a) {
 #define SPK_STRIDED_CASE(NN)                                                                       \
     case NN: {                                                                                     \
-        constexpr int TC = strided_tcols(NN);                                                      \
+        constexpr int TC = strided_tcols_t<T>(NN);                                                 \
         if constexpr (AXIS == 1 && TC > 2) {                                                       \
             if (ncols == 2) {                                                                      \
                 dim3 grid((unsigned)nouter, a.nfields);                                           \
                 k_pass_strided<T, NN, DIR, AXIS, 2><<<grid, 2 * strided_tpl(NN), 0, ctx->stream>>>(   \
-                    (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, 1, a.fstride_in, \
-                    a.fstride_out, twp, (T)a.scale, ctx->gate);                                    \
+                    (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, nouter,         \
+                    a.fstride_in, a.fstride_out, twp, (T)a.scale, ctx->gate);                      \
                 break;                                                                             \
             }                                                                                      \
         }                                                                                          \
-        const int ntiles = (int)((ncols + TC - 1) / TC);                                           \
-        dim3 grid((unsigned)ntiles * nouter, a.nfields);                                          \
-        k_pass_strided<T, NN, DIR, AXIS><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(               \
-            (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, ntiles, a.fstride_in, \
+        const long long ntiles = (ncols * nouter + TC - 1) / TC;                                   \
+        dim3 grid((unsigned)ntiles, a.nfields);                                                    \
+        k_pass_strided<T, NN, DIR, AXIS, TC><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(         \
+            (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, nouter, a.fstride_in, \
             a.fstride_out, twp, (T)a.scale, ctx->gate);                                            \
         break;                                                                                     \
     }

@@ -9,7 +9,7 @@ bool fused_supported_nx(int nx) { return nx >= 32 && nx <= 1024 && is_pow2(nx); 
 
 template <class T, int N, int LY, int EF>
 static void launch_ly_mode(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T> *P, cpx<double> *U,
-                           cpx<double> *S, cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts) {
+                           cpx<T> *S, cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts) {
     switch (mode) {
         case 1: k_fwdx_epilogue<T, N, LY, 1, EF><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts); break;
         case 2: k_fwdx_epilogue<T, N, LY, 2, EF><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts); break;
@@ -25,7 +25,8 @@ static int launch_n(int ly, int mode, hipStream_t st, const void *P, void *U, vo
     dim3 grid((unsigned)((ncols + TC - 1) / TC), ngroups);
     const int bs = TC * fx_tpl(N, EF);
     const cpx<T> *p = (const cpx<T> *)P;
-    cpx<double> *u = (cpx<double> *)U, *s = (cpx<double> *)S;
+    cpx<double> *u = (cpx<double> *)U;
+    cpx<T> *s = (cpx<T> *)S;  // the RK3 increment register, bulk precision
     cpx<T> *w = (cpx<T> *)W;
     const cpx<T> *t = (const cpx<T> *)tw;
     switch (ly) {

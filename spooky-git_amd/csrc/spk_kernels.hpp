@@ -489,11 +489,25 @@ __host__ __device__ constexpr int fx_e(int n, int ef) { return line_e(n, ef); }
 __host__ __device__ constexpr int fx_tpl(int n, int ef) { return n / fx_e(n, ef); }
 __host__ __device__ constexpr int fx_tcols(int n, int ef) { return fx_tpl(n, ef) >= 512 ? 1 : 512 / fx_tpl(n, ef); }
 
+// RK3 registers of the fused path.  Wray's scheme (hydro_mhd_advance.cu:114-164)
+//   U1 = U0 + g1 dt L0, S1 = U1 + x1 dt L0, U2 = S1 + g2 dt L1, S2 = U2 + x2 dt L1,
+//   U3 = S2 + g3 dt L2
+// is kept with the second register holding the stage increment R = dt L in the
+// bulk precision instead of S in FP64:
+//   stage 1: reads U0, writes R0 (U1 = U0 + g1 R0 is formed where it is needed)
+//   stage 2: reads U0, R0; current state U1 = U0 + cu R0 (cu = g1), base
+//            S1 = U0 + cs R0 (cs = g1 + x1); writes U2 = S1 + g2 R1 and R1
+//   stage 3: reads U2, R1; base S2 = U2 + cs R1 (cs = x2); writes U3 = S2 + g3 dt L2
+// Every stage's state is formed from the rounded increments, so the nonlinear and
+// linear terms of a stage see the same state.  The FP64 state accumulation is
+// unchanged; with FP32 bulk the increments carry FP32 rounding, as the products
+// they come from already do.
 struct FusedConst {
     EpiConst e;
     int project;    // apply CleanDivergence to the updated u and B (stage 3)
     int inv;        // write the next stage's inverse x pass of the updated fields to W
     double scale;   // 1/(NX NY NZ) of the next inverse pass
+    double cu, cs;  // coefficients of the stored increment in the current state and the base
 };
 
 // One field group of k_fwdx_epilogue: GRP 0 velocity (5 products T_ij), 1
@@ -513,7 +527,7 @@ struct FusedConst {
 // memory operation sits behind a branch.  Element offsets inside a field fit
 // 32 bits and are added to wave-uniform field bases.
 template <class T, int N, int LY, int MODE, int EF, int GRP>
-__device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S,
+__device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ R,
                                          cpx<T> *__restrict__ W, const FusedConst &fc,
                                          const cpx<T> *__restrict__ tw, cpx<T> *twl, cpx<T> *ldsc, cpx<T> *pbuf,
                                          const TimeState *__restrict__ ts, int tid, int nt, int t, long long col) {
@@ -623,15 +637,17 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     // i + PD is loaded while point i is computed; the schedule barrier at the end
     // of each point keeps the compiler from hoisting all E points' loads at once.
     const double dt = ts->dt;
-    const double ga = c.gam * dt, xb = c.xi * dt;
+    const double gam = c.gam, ga = c.gam * dt;
+    const double cu = fc.cu, cs = fc.cs;
     constexpr int PD = 1;
-    cpx<double> su[E][NCOMP], ss[E][NCOMP];
+    cpx<double> su[E][NCOMP];
+    cpx<T> sr[E][NCOMP];
     auto load_state = [&](int i) {
         const unsigned pi = ucol + (unsigned)(t + i * TPL) * ncols;
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
             su[i][q] = (U + (F0 + q) * N0)[pi];
-            if constexpr (MODE != 1) ss[i][q] = (S + (F0 + q) * N0)[pi];
+            if constexpr (MODE != 1) sr[i][q] = (R + (F0 + q) * N0)[pi];
         }
     };
 #pragma unroll
@@ -644,13 +660,26 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         double kx, kyi, kzi, m;
         kvec_at(c.kg, n, j, kk, kx, kyi, kzi, m);
         const double k2 = kx * kx + ky * ky + kz * kz;
+        // the stage's current state (diffusion) and the base of its update
+        cpx<double> uc[NCOMP], base[NCOMP];
+#pragma unroll
+        for (int q = 0; q < NCOMP; q++) {
+            if constexpr (MODE == 1) {
+                uc[q] = su[i][q];
+                base[q] = su[i][q];
+            } else {
+                const cpx<double> r = cconv<double>(sr[i][q]);
+                uc[q] = MODE == 2 ? cpx<double>{su[i][q].x + cu * r.x, su[i][q].y + cu * r.y} : su[i][q];
+                base[q] = cpx<double>{su[i][q].x + cs * r.x, su[i][q].y + cs * r.y};
+            }
+        }
         cpx<double> d[NCOMP];
         if constexpr (GRP == 0) {
 #pragma unroll
             for (int q = 0; q < 3; q++) d[q] = mul_minus_im(m, acc[q][i]);
             const double f = -c.nu * k2;
 #pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::VX + q, d[q], su[i][q], f);
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::VX + q, d[q], uc[q], f);
             // GradPseudoPressure
             const cpx<double> sd3 = kdot3(kx, d[0], ky, d[1], kz, d[2]);
             const cpx<double> div = {-sd3.y, sd3.x};
@@ -664,18 +693,23 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             for (int q = 0; q < 3; q++) d[q] = mul_plus_im(m, acc[q][i]);
             const double f = -c.nu_m * k2;
 #pragma unroll
-            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::BX + q, d[q], su[i][q], f);
+            for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::BX + q, d[q], uc[q], f);
         } else {
             d[0] = mul_minus_im(m, acc[0][i]);
-            d[0] = diffuse(c.diff_mask, L::TH, d[0], su[i][0], -c.nu_th * k2);
+            d[0] = diffuse(c.diff_mask, L::TH, d[0], uc[0], -c.nu_th * k2);
         }
         cpx<double> un[NCOMP];
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
-            const cpx<double> base = (MODE == 1) ? su[i][q] : ss[i][q];
-            un[q] = cpx<double>{base.x + ga * d[q].x, base.y + ga * d[q].y};
-            if constexpr (MODE < 3)
-                (S + (F0 + q) * N0)[pi] = cpx<double>{un[q].x + xb * d[q].x, un[q].y + xb * d[q].y};
+            if constexpr (MODE < 3) {
+                // the increment in the bulk precision, and the update from that rounded value
+                const cpx<T> rq = cconv<T>(cscale(d[q], dt));
+                (R + (F0 + q) * N0)[pi] = rq;
+                const cpx<double> r = cconv<double>(rq);
+                un[q] = cpx<double>{base[q].x + gam * r.x, base[q].y + gam * r.y};
+            } else {
+                un[q] = cpx<double>{base[q].x + ga * d[q].x, base[q].y + ga * d[q].y};
+            }
         }
         if constexpr (GRP < 2) {
             if (fc.project) {
@@ -693,7 +727,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         }
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
-            (U + (F0 + q) * N0)[pi] = un[q];
+            if constexpr (MODE != 1) (U + (F0 + q) * N0)[pi] = un[q];  // stage 1 keeps U0
             acc[q][i] = un[q];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -715,7 +749,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 
 template <class T, int N, int LY, int MODE, int EF>
 __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
-    const cpx<T> *P, cpx<double> *__restrict__ U, cpx<double> *__restrict__ S, cpx<T> *__restrict__ W,
+    const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ S, cpx<T> *__restrict__ W,
     FusedConst fc, const cpx<T> *__restrict__ tw, const TimeState *__restrict__ ts) {
     using L = Layout<LY>;
     constexpr int E = fx_e(N, EF);
