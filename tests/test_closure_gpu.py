@@ -153,7 +153,10 @@ def device_fields(ctx, f):
     return buf
 
 
-def test_forward_matches_numpy_and_known_answers(ctx):
+@pytest.mark.parametrize("n", [N, 128])
+def test_forward_matches_numpy_and_known_answers(ctx, n):
+    """At 32^3 and at the closure's own 128^3 grid."""
+    N = n
     rng = np.random.default_rng(1)
     cl = spk.Closure(ctx, N)
     assert cl.nparams == sum(27 * 32 * c + 32 for c in spk.Closure.CIN)
@@ -188,14 +191,26 @@ def test_forward_matches_numpy_and_known_answers(ctx):
     for _ in range(3):
         a = bf(gelu(a))
     assert rel(tau[:3], a) < 1e-3 and np.all(tau[3:] == 0)
-    # random network against the numpy restatement
+    # random network against the numpy restatement: the whole grid at 32^3; at
+    # 128^3 a 24^3 block that wraps around the periodic boundary, from a 32^3
+    # crop of the input (each of the 4 layers consumes one halo point)
     p = random_params(rng)
     cl.set_params(p)
     cl.set_scales(0.7, scales)
     cl.forward(du.ptr, dtau.ptr)
     tau = dtau.download(np.float64, (6, N, N, N + 2))[..., :N]
-    ref = np.moveaxis(reference(p, u, 0.7), -1, 0) * np.array(scales)[:, None, None, None]
-    assert rel(tau, ref) < 1e-2, rel(tau, ref)
+    sc = np.array(scales)[:, None, None, None]
+    if N <= 32:
+        ref = np.moveaxis(reference(p, u, 0.7), -1, 0) * sc
+        assert rel(tau, ref) < 1e-2, rel(tau, ref)
+    else:
+        m, h = 24, 4
+        idx = [(np.arange(-h, m + h) + s0) % N for s0 in (N - 10, 50, N - 3)]
+        crop = u[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+        ref = np.moveaxis(reference(p, crop, 0.7), -1, 0)[:, h:h + m, h:h + m, h:h + m] * sc
+        inner = [ix[h:h + m] for ix in idx]
+        got = tau[:, inner[0]][:, :, inner[1]][:, :, :, inner[2]]
+        assert rel(got, ref) < 1e-2, rel(got, ref)
     cl.destroy()
 
 

@@ -118,6 +118,56 @@ def test_restart_reproduces_uninterrupted_run(tmp_path):
         assert np.max(np.abs(fa[k] - fb[k])) < 1e-12
 
 
+def test_heat_diffusion(tmp_path):
+    """tests/heat-diffusion (check-heat-diffusion.py, test_rk3_heatdiff.py): the
+    double top-hat th_0 of tanh edges (a = 0.01) diffusing with nu_th = 1e-3 on
+    1024x2x2, run with -r 0 to t_final = 10 with a snapshot at t = 5.  Pinned by
+    the exact RK3 amplification 1 + z + z^2/2 + z^3/6 (z = -nu_th kx^2 dt) of every
+    mode over the dt sequence of the heat-equation branch of compute_dt
+    (dt = cfl cfl_par / (nu_th kxmax^2), compute_dt.cu:40-57, clamped to the
+    snapshot and final times, :288-293).  Every y-z column uniform (:150-155).
+    Tolerance: L1 (mean absolute) error 1e-12.  The reference's 1e-15
+    (check-heat-diffusion.py:29, :160) compares two codes with the same operation
+    order; over these 45818 steps the product of exact amplifications and Wray's
+    two-register form, both in numpy FP64, already differ by 1.2e-13 in L1, so a
+    bound below that would test rounding order, not the algorithm."""
+    nx, chi, cfl, cfl_par, t_final, t_out = 1024, 1e-3, 1.5, 1.5, 10.0, 5.0
+    out = str(tmp_path)
+    x = -0.5 + np.arange(nx) / nx
+    th0 = 1 + 0.5 * (np.tanh((x + 0.375) / 0.01) - np.tanh((x + 0.125) / 0.01)) \
+        + 0.5 * (np.tanh((x - 0.125) / 0.01) - np.tanh((x - 0.375) / 0.01))
+    write_ic(out, {"th": np.broadcast_to(th0[:, None, None], (nx, 2, 2)).copy()})
+    run(os.path.join(CFG, "heat-diffusion"), out, "-r", "0")
+    snaps = read_all(out, ["th"], (nx, 2, 2))
+    assert [round(h["t_save"], 12) for h, _ in snaps] == [0.0, 5.0, 10.0]
+    # the expected states: every mode times the RK3 amplification of each step
+    kx = 2 * np.pi * np.fft.rfftfreq(nx, 1.0 / nx)
+    kxmax = 2 * np.pi * (nx // 2 - 1)
+    dt0 = cfl * cfl_par / (chi * kxmax ** 2)
+    th_hat = np.fft.rfft(th0)
+    t, t_last, step, expect = 0.0, 0.0, 0, {}
+    while t < t_final:
+        dt = dt0
+        if t + dt > t_final:
+            dt = t_final - t
+        elif t + dt - t_last > t_out:
+            dt = t_out - t + t_last
+        z = -chi * kx ** 2 * dt
+        th_hat = th_hat * (1 + z + z * z / 2 + z * z * z / 6)
+        t += dt
+        step += 1
+        if t - t_last >= t_out:
+            t_last += t_out
+            expect[round(t_last, 9)] = (step, np.fft.irfft(th_hat, n=nx))
+    for hdr, f in snaps[1:]:
+        th = f["th"]
+        assert np.all(th == th[:, :1, :1])  # uniform along y and z
+        st, ref = expect[round(hdr["t_save"], 9)]
+        assert hdr["step"] == st, (hdr, st)
+        l1 = np.mean(np.abs(th[:, 0, 0] - ref))
+        assert l1 < 1e-12, (hdr["t_save"], l1)
+
+
 @pytest.mark.parametrize("nx", [32, 128, 512])
 def test_heat_diffusion_rkl2(tmp_path, nx):
     """tests/heat-diffusion-sts, rkl2 column (check-heat-diffusion-sts.py:60-122):

@@ -170,3 +170,57 @@ def test_ensemble_refuses_shearing_box(ctx):
     p = dict(CASES["shear_mhd"]["params"])
     with pytest.raises(RuntimeError):
         spk.Ensemble([p], "f64")
+
+
+def boxes_at_size(seed=2026):
+    """One box of each kind of the ensemble workload at its own size: MHD 64^3 and
+    128^3 (2 pi box, CFL 0.5) and a 512x512x2 Boussinesq-MHD box with anisotropic
+    conduction and RKL2 on theta (problems/hbi2d physics), with nu, eta drawn
+    log-uniform in [1e-4, 1e-2] and chi_par uniform in [1e-3, 1e-1]."""
+    rng = np.random.default_rng(seed)
+    T = 2 * np.pi
+    out = []
+    for n in (64, 128):
+        nu, eta = 10.0 ** rng.uniform(-4, -2, 2)
+        out.append(dict(nx=n, ny=n, nz=n, incompressible=1, mhd=1, lx=T, ly=T, lz=T, nu=float(nu), nu_m=float(eta),
+                        cfl=0.5, t_final=1e9, toutput_flow=1e9))
+    nu, eta = 10.0 ** rng.uniform(-4, -2, 2)
+    out.append(ini.hbi_params(512, nu=float(nu), eta=float(eta), chi=float(rng.uniform(1e-3, 1e-1))))
+    return out
+
+
+def state_at_size(p, seed):
+    shape = (p["nx"], p["ny"], p["nz"])
+    if p.get("boussinesq"):
+        return ini.hbi_noise(shape, seed=seed)
+    return ini.mhd_random(shape, seed=seed)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ensemble_boxes_at_size_vs_oracle(ctx, prec):
+    """The ensemble workload's box kinds at their own sizes, advanced together
+    (graph on, 4 steps: two RKL2 cycles on the HBI box) against the FP64 oracle
+    box by box: state rel-L2 <= 1e-10 (FP64 bulk) / 2e-4 (FP32 bulk)."""
+    bx = boxes_at_size()
+    states = [state_at_size(p, 7000 + i) for i, p in enumerate(bx)]
+    ens = spk.Ensemble(bx, prec, lanes=3)
+    for i, st in enumerate(states):
+        ens.set_fields(i, st)
+    ens.run(4)
+    got = [ens.get_fields(i) for i in range(len(bx))]
+    tms = [ens.time(i) for i in range(len(bx))]
+    ens.destroy()
+    tol = 1e-10 if prec == "f64" else 2e-4
+    for i, (p, st) in enumerate(zip(bx, states)):
+        print("# box %d: %dx%dx%d vs oracle" % (i, p["nx"], p["ny"], p["nz"]), flush=True)
+        sim = orc.Sim(**p)
+        sim.set_fields(st)
+        sim.main_loop(4)
+        e = relerr(got[i], sim.fields)
+        assert e < tol, (i, prec, e)
+        assert tms[i]["step"] == 4
+        assert abs(tms[i]["t"] - sim.time()["t"]) <= (1e-13 if prec == "f64" else 1e-5) * sim.time()["t"]
+        if p.get("supertimestepping"):
+            t = sim.time()
+            assert rkl_stages(t["dt"] / t["dt_sts"]) > 2  # the RKL2 cycles are multi-stage
+        del sim
