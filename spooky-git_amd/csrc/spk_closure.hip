@@ -717,6 +717,7 @@ extern "C" int spk_closure_create(spk_ctx ctx, int n, spk_closure *out) {
         delete c;
         return rc;
     }
+    ctx_retain(ctx);
     *out = c;
     return pack_weights(c);
 }
@@ -725,7 +726,9 @@ extern "C" int spk_closure_destroy(spk_closure c) {
     if (!c) return SPK_OK;
     (void)hipStreamSynchronize(c->ctx->stream);
     closure_free(c);
+    spk_ctx cx = c->ctx;
     delete c;
+    ctx_release(cx);
     return SPK_OK;
 }
 

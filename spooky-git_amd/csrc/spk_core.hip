@@ -14,6 +14,15 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
+spk_ctx ctx_retain(spk_ctx ctx) {
+    ctx->refs.fetch_add(1, std::memory_order_relaxed);
+    return ctx;
+}
+
+void ctx_release(spk_ctx ctx) {
+    if (ctx && ctx->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete ctx;
+}
+
 }  // namespace spk
 
 extern "C" int spk_abi_version(void) { return SPK_ABI_VERSION; }
@@ -36,7 +45,7 @@ extern "C" int spk_ctx_create(int device, void *stream, spk_ctx *out) {
 }
 
 extern "C" int spk_ctx_destroy(spk_ctx ctx) {
-    delete ctx;
+    spk::ctx_release(ctx);
     return SPK_OK;
 }
 

@@ -311,6 +311,7 @@ extern "C" int spk_fft_plan_create(spk_ctx ctx, int nx, int ny, int nz, int prec
         delete p;
         return r;
     }
+    ctx_retain(ctx);
     *out = p;
     return SPK_OK;
 }
@@ -318,7 +319,9 @@ extern "C" int spk_fft_plan_create(spk_ctx ctx, int nx, int ny, int nz, int prec
 extern "C" int spk_fft_plan_destroy(spk_fft_plan p) {
     if (!p) return SPK_OK;
     spk::twiddles_destroy(&p->tw);
+    spk_ctx c = p->ctx;
     delete p;
+    spk::ctx_release(c);
     return SPK_OK;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -13,6 +14,11 @@
 struct spk_ctx_s {
     int device;
     hipStream_t stream;
+    // the caller's handle plus one per solver, plan and closure made on this
+    // context: spk_ctx_destroy drops the caller's, and the context is deleted
+    // when the last holder lets go (a solver destroyed after its context still
+    // synchronizes on a live stream)
+    std::atomic<int> refs{1};
     // When set, transform kernels launched on this context return at once unless
     // *gate is non-zero (device-sized loops: the stages of an RKL2 cycle beyond
     // the stage count the device computed for this step)
@@ -23,6 +29,10 @@ namespace spk {
 
 // thread-local error text behind spk_last_error()
 void set_error(const char *fmt, ...);
+
+// context reference counting (spk_core.hip)
+spk_ctx ctx_retain(spk_ctx ctx);
+void ctx_release(spk_ctx ctx);
 
 // Evaluates a HIP call; on failure records file:line and the HIP message and
 // returns SPK_ERR_HIP from the enclosing function.

@@ -1478,6 +1478,7 @@ extern "C" int spk_solver_create(spk_ctx ctx, const spk_params *p, int bulk, spk
         delete s;
         return r;
     }
+    ctx_retain(ctx);
     *out = s;
     return SPK_OK;
 }
@@ -1486,7 +1487,9 @@ extern "C" int spk_solver_destroy(spk_solver s) {
     if (!s) return SPK_OK;
     (void)hipStreamSynchronize(s->ctx->stream);
     free_all(s);
+    spk_ctx c = s->ctx;
     delete s;
+    ctx_release(c);
     return SPK_OK;
 }
 

@@ -151,7 +151,8 @@ def test_aniso_flux_against_oracle(ctx):
     pr = Pair(ctx, name)
     p = pr.p
     th = pr.h.vars["TH"]
-    rc(L().spk_aniso_flux(pr.c.h, pr.F.ptr, None, pr.D.ptr))
+    # complex_dTheta is the theta slot of the dfields (physics.cu:214, 320)
+    rc(L().spk_aniso_flux(pr.c.h, pr.F.ptr, None, ctypes.c_void_p(pr.D.ptr.value + th * pr.fb)))
     got = pr.D.download(np.complex128, (pr.nf,) + pr.shape_c)
     d = {}
     for tag, ra in (("on", p["reynolds_ani"]), ("off", 1e300)):
