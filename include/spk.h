@@ -262,6 +262,18 @@ int spk_aniso_flux(spk_solver s, const void *complex_fields, double *real_buffer
 int spk_maxabs(spk_solver s, const double *real_field, double *out_dev);
 /* computeEnergy(data_type *vcomplex) (spooky_output_functions.cu:23-55); synchronizes */
 int spk_energy_ptr(spk_solver s, const void *complex_field, double *out);
+/* No counterpart in the reference.  spk_energy_ptr for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): computeEnergy of one spectral field,
+ * [NX][NY][NZ/2+1] complex FP64 (16-byte aligned), into one device double
+ * out_dev (8-byte aligned).  Needs no solver; allocates nothing; asynchronous
+ * on the context stream, no synchronization.  Its reduction partials live in a
+ * static device array of SPK_NOALLOC_PARTS doubles (8 KiB) that each GPU holds
+ * until the process ends; a call whose partials would not fit is rejected
+ * (the reduction strides over the field, so no grid needs more).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define SPK_NOALLOC_PARTS 1024
+int spk_energy_noalloc(spk_ctx ctx, int nx, int ny, int nz, const void *complex_field, double *out_dev);
 /* computeEnstrophy(data_type *complex_vecField) (spooky_output_functions.cu:57-86) */
 int spk_enstrophy_ptr(spk_solver s, const void *complex_vec, double *out);
 /* twoFieldCorrelation(scalar_type *v1, scalar_type *v2) (spooky_output_functions.cu:188-235) */

@@ -145,6 +145,7 @@ SIGNATURES = {
     "spk_aniso_flux": [_VP, _VP, _VP, _VP],
     "spk_maxabs": [_VP, _VP, _VP],
     "spk_energy_ptr": [_VP, _VP, _VP],
+    "spk_energy_noalloc": [_VP, _I, _I, _I, _VP, _VP],
     "spk_enstrophy_ptr": [_VP, _VP, _VP],
     "spk_correlation_ptr": [_VP, _VP, _VP, _VP],
     "spk_spectrum_ptr": [_VP, _VP, _VP, _VP, _I, _VP],

@@ -37,6 +37,7 @@ extern "C" int spk_ctx_create(int device, void *stream, spk_ctx *out) {
     SPK_HIP(hipGetDeviceCount(&n));
     SPK_REQUIRE(device >= 0 && device < n, SPK_ERR_ARG, "spk_ctx_create: device %d of %d", device, n);
     SPK_HIP(hipSetDevice(device));
+    SPK_TRY(spk::noalloc_setup(device));
     spk_ctx c = new spk_ctx_s();
     c->device = device;
     c->stream = (hipStream_t)stream;

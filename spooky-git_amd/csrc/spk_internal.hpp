@@ -121,6 +121,10 @@ int launch_pass_z_r2c(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, 
 int launch_pass_z_c2r(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int tin, int tout,
                       const PassArgs &a);
 
+// looks up, with `device` current, the address of that device's copy of the
+// static scratch array behind spk_energy_noalloc (spk_solver.hip); once per device
+int noalloc_setup(int device);
+
 // solver internals the ensemble needs (spk_solver.hip)
 spk_ctx solver_ctx(spk_solver s);
 int solver_host_step(spk_solver s);

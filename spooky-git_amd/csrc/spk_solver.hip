@@ -14,7 +14,9 @@
 #include "spk_kernels.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2456,4 +2458,77 @@ extern "C" int spk_spectrum_ptr(spk_solver s, const void *complex_a, const void 
     if (nbins) *nbins = s->nbins;
     return spectrum_of(s, static_cast<const cpx<double> *>(complex_a), static_cast<const cpx<double> *>(complex_b),
                        out, cap);
+}
+
+// ==========================================================================
+// Energy of a spectral field without allocating
+// ==========================================================================
+// spk_energy_noalloc is spk_energy_ptr for callers that cannot allocate device
+// memory.  Its reduction partials live in `scratch`, a static device array
+// every GPU holds once for the life of the process.  In host code the array's
+// name is not a device address: noalloc_setup looks the address up on each
+// device once (spk_ctx_create calls it with the device current) and the entry
+// point passes that address to the kernels.  Every call on one GPU shares that
+// GPU's array; calls that overlap on one GPU race.  Each call writes partials
+// 0 .. nb-1 before its final sum reads exactly those, so nothing needs
+// clearing between calls.
+
+static __device__ __align__(256) double scratch[SPK_NOALLOC_PARTS];
+static_assert(SPK_NOALLOC_PARTS >= RED_BLOCKS, "the scratch array holds every reduction partial");
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+}  // namespace
+
+namespace spk {
+
+int noalloc_setup(int device) {
+    SPK_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, SPK_ERR_ARG, "noalloc_setup: device %d", device);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return SPK_OK;
+    int cur = -1;
+    SPK_HIP(hipGetDevice(&cur));
+    SPK_REQUIRE(cur == device, SPK_ERR_STATE, "noalloc_setup: device %d is not current (%d is)", device, cur);
+    void *p = nullptr;
+    SPK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SPK_REQUIRE(p != nullptr, SPK_ERR_HIP, "noalloc_setup: no address for the scratch array on device %d", device);
+    g_noalloc_ptr[device] = static_cast<double *>(p);
+    return SPK_OK;
+}
+
+}  // namespace spk
+
+extern "C" int spk_energy_noalloc(spk_ctx ctx, int nx, int ny, int nz, const void *complex_field, double *out_dev) {
+    SPK_REQUIRE(ctx, SPK_ERR_ARG, "spk_energy_noalloc: null ctx");
+    SPK_REQUIRE(nx > 0 && ny > 0 && nz > 0, SPK_ERR_ARG, "spk_energy_noalloc: bad grid %d x %d x %d", nx, ny, nz);
+    SPK_REQUIRE(complex_field && ((uintptr_t)complex_field & 15u) == 0, SPK_ERR_ARG,
+                "spk_energy_noalloc: complex_field must be a 16-byte aligned device pointer");
+    SPK_REQUIRE(out_dev && ((uintptr_t)out_dev & 7u) == 0, SPK_ERR_ARG,
+                "spk_energy_noalloc: out_dev must be an 8-byte aligned device pointer");
+    SPK_REQUIRE(ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES, SPK_ERR_ARG, "spk_energy_noalloc: device %d",
+                ctx->device);
+    double *part;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        part = g_noalloc_ptr[ctx->device];
+    }
+    SPK_REQUIRE(part, SPK_ERR_STATE, "spk_energy_noalloc: scratch address of device %d not set up", ctx->device);
+    const Grid3 g = make_grid(nx, ny, nz);
+    const unsigned nb = red_grid(g.ntc);
+    SPK_REQUIRE(nb <= (unsigned)SPK_NOALLOC_PARTS, SPK_ERR_ARG,
+                "spk_energy_noalloc: %u reduction partials exceed the %d-entry scratch array", nb, SPK_NOALLOC_PARTS);
+    KGeom kg{};
+    kg.nx = nx;
+    kg.ny = ny;
+    kg.nzh = g.nzh;
+    k_spec_sum<<<nb, 256, 0, ctx->stream>>>(static_cast<const cpx<double> *>(complex_field), g.ntc, kg, 0, part);
+    SPK_LAUNCH_CHECK();
+    const double nt = (double)nx * ny * nz;
+    k_final_sum<<<1, 1024, 0, ctx->stream>>>(part, (int)nb, 1.0 / (nt * nt), out_dev);
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
 }
