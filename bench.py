@@ -22,8 +22,9 @@ Beside the timed line (not part of `value`):
                              FP64 RKL2 on theta: steps/s and stages per cycle;
   ensemble                   48 MHD boxes at 64^3 + 16 at 128^3 + 32 Boussinesq-MHD
                              boxes at 512x512x2 with RKL2 conduction, FP32 bulk,
-                             advanced together (spk_ensemble): steps of every box
-                             per second, and representative boxes against FP64.
+                             one batch per shape (spk_ensemble): steps of every box
+                             per second, algorithmic GB/s, every box against an
+                             FP64 ensemble and one box per kind against the oracle.
 """
 import argparse
 import importlib
@@ -54,6 +55,10 @@ def parse():
     ap.add_argument("--no-extra", dest="extra", action="store_false",
                     help="skip the Taylor-Green 64^3 and MTI legs")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-acc-steps", type=int, default=10,
+                    help="steps of the FP32-vs-FP64 ensemble accuracy check (0: skip)")
+    ap.add_argument("--ensemble-split", type=int, default=-1,
+                    help="batches per shape group (0: box by box; -1: autotuned)")
     ap.add_argument("--closure-spinup", type=float, default=2.0)
     ap.add_argument("--closure-iters", type=int, default=3000)
     ap.add_argument("--closure-dns", type=int, default=512)
@@ -358,11 +363,32 @@ def box_state(ini, p, seed):
     return ini.mhd_random(shape, seed=seed)
 
 
-def ensemble_leg(spk, ini, ctx, steps):
-    """The ensemble in FP32 bulk: lane count autotuned, then `steps` steps of
-    every box timed (host clock around run + sync) with the graph, launched box
-    by box without it, and box by box on one stream; representative boxes
-    against the FP64 path from the same initial state after the same steps."""
+def ensemble_bytes(p, bulk_bytes):
+    """Algorithmic HBM bytes of one main-loop step of a box, counted as for the
+    256^3 step: every 3D transform one sweep of its fields (2 S_bulk per field,
+    read + write) and the FP64 RK3 state traffic (U read/write, increment
+    register) of the fused back end; RKL2 stages as 6 transforms + 6 S_64 of
+    state traffic + 3 S_bulk of real-space B per stage (taken from the box's own
+    stage count)."""
+    nzh = p["nz"] // 2 + 1
+    ntc = p["nx"] * p["ny"] * nzh
+    s64, sb = 16 * ntc, bulk_bytes * ntc
+    nf = 3 + (3 if p.get("mhd") else 0) + (1 if p.get("boussinesq") else 0)
+    nin = nf
+    nout = 5 + (3 if p.get("mhd") else 0) + (3 if p.get("boussinesq") else 0)
+    transforms = 3 * (nin + nout)
+    return 2 * transforms * sb + 7 * nf * s64, 6 * 2 * sb + 6 * s64 + 3 * sb
+
+
+def ensemble_leg(spk, ini, ctx, steps, acc_steps=10, split=-1):
+    """The ensemble in FP32 bulk.  Boxes of one shape run as one batch (one
+    launch per kernel for all of them); the batch split is autotuned, then
+    `steps` steps of every box are timed (host clock around run + sync) with the
+    graph, without it, and box by box over 16 lanes (the unbatched launch).
+    Accuracy: from the same initial states, an FP32-bulk and an FP64-bulk
+    ensemble advance acc_steps steps and every box's state and energy are
+    compared (max over the boxes of each kind); the FP64 ensemble's first box of
+    each kind is checked against the FP64 oracle."""
     import numpy as np
     boxes, kinds = ensemble_boxes(ini)
     t0 = time.perf_counter()
@@ -374,8 +400,12 @@ def ensemble_leg(spk, ini, ctx, steps):
     setup_s = time.perf_counter() - t0
     print("# ensemble: %d boxes created and initialized in %.1f s" % (len(boxes), setup_s), file=sys.stderr,
           flush=True)
-    lanes, _ = ens.tune(probe=2)
-    print("# ensemble: %d lanes after tuning" % lanes, file=sys.stderr, flush=True)
+    if split < 0:
+        split, _ = ens.tune(probe=2)
+    else:
+        ens.set_split(split)
+    layout = ens.layout()
+    print("# ensemble: %s after tuning" % layout, file=sys.stderr, flush=True)
 
     def timed(n):
         ens.run(2)
@@ -386,38 +416,71 @@ def ensemble_leg(spk, ini, ctx, steps):
         return (time.perf_counter() - t1) * 1000.0 / n
 
     graph_ms = timed(steps)
-    ens.set_graph(False)
-    eager_ms = timed(steps)
-    ens.set_lanes(1)
-    one_stream_ms = timed(steps)
-    print("# ensemble: %.2f ms per step of every box (graph, %d lanes); %.2f without the graph; %.2f on one "
-          "stream" % (graph_ms, lanes, eager_ms, one_stream_ms), file=sys.stderr, flush=True)
-    ens.set_lanes(lanes)
-    ens.set_graph(True)
-    cells = sum(p["nx"] * p["ny"] * p["nz"] for p in boxes)
     hbi = [i for i, k in enumerate(kinds) if k.startswith("hbi")]
     stages = [rkl2_stages(ens.time(i)["dt"], ens.time(i)["dt_sts"]) for i in hbi]
-    acc = {}
-    for kind in dict.fromkeys(kinds):
-        i = kinds.index(kind)
-        nstep = int(ens.time(i)["step"])
-        got = ens.get_fields(i)
-        ref = spk.Solver(ctx, "f64", **boxes[i])
-        ref.set_fields(states[i])
-        ref.main_loop(nstep)
-        r = ref.get_fields()
-        e_ref = sum(ref.energy(f) for f in range(ref.nf))
-        e_got = sum(ens.boxes[i].energy(f) for f in range(ref.nf))
-        acc[kind] = dict(box=i, steps=nstep, rel_energy_err=abs(e_got - e_ref) / abs(e_ref),
-                         rel_l2_state=float(np.linalg.norm(got - r) / np.linalg.norm(r)))
-        ref.destroy()
+    ens.set_graph(False)
+    eager_ms = timed(steps)
+    ens.set_graph(True)
+    box_ms = None
+    if split != 0:
+        ens.set_lanes(16)
+        box_ms = timed(steps)
+    print("# ensemble: %.2f ms per step of every box (%s, graph); %.2f without the graph; %s box by box over "
+          "16 lanes (graph)" % (graph_ms, layout, eager_ms, box_ms), file=sys.stderr, flush=True)
     ens.destroy()
-    return dict(boxes={k: kinds.count(k) for k in dict.fromkeys(kinds)}, bulk="f32", lanes=lanes, steps=steps,
+    # algorithmic bytes per step of every box (RKL2 stage count of each HBI box at the timed point)
+    step_b = 0.0
+    st_iter = iter(stages)
+    for p in boxes:
+        b, per_stage = ensemble_bytes(p, 4)
+        step_b += b + (per_stage * next(st_iter) if p.get("supertimestepping") else 0.0)
+    cells = sum(p["nx"] * p["ny"] * p["nz"] for p in boxes)
+
+    # accuracy: FP32 bulk against FP64 bulk, box by box, and FP64 against the oracle
+    if acc_steps <= 0:
+        return dict(boxes={k: kinds.count(k) for k in dict.fromkeys(kinds)}, layout=layout,
+                    ms_per_ensemble_step=round(graph_ms, 3), algorithmic_gb_per_ensemble_step=round(step_b / 1e9, 3),
+                    achieved_gbs=round(step_b / (graph_ms * 1e-3) / 1e9, 1))
+    e32 = spk.Ensemble(boxes, "f32")
+    e64 = spk.Ensemble(boxes, "f64")
+    for i, st in enumerate(states):
+        e32.set_fields(i, st)
+        e64.set_fields(i, st)
+    e32.run(acc_steps)
+    e64.run(acc_steps)
+    acc = {}
+    for i, k in enumerate(kinds):
+        a, r = e32.get_fields(i), e64.get_fields(i)
+        ea = sum(e32.boxes[i].energy(f) for f in range(e32.boxes[i].nf))
+        er = sum(e64.boxes[i].energy(f) for f in range(e64.boxes[i].nf))
+        d = acc.setdefault(k, dict(boxes=0, max_rel_l2_state=0.0, max_rel_energy_err=0.0))
+        d["boxes"] += 1
+        d["max_rel_l2_state"] = max(d["max_rel_l2_state"], float(np.linalg.norm(a - r) / np.linalg.norm(r)))
+        d["max_rel_energy_err"] = max(d["max_rel_energy_err"], abs(ea - er) / abs(er))
+    oracle = {}
+    try:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle as orc
+        for k in dict.fromkeys(kinds):
+            i = kinds.index(k)
+            sim = orc.Sim(**boxes[i])
+            sim.set_fields(states[i])
+            sim.main_loop(acc_steps)
+            r = sim.fields
+            oracle[k] = dict(box=i, rel_l2_state_f64=float(np.linalg.norm(e64.get_fields(i) - r) / np.linalg.norm(r)))
+    except OSError as ex:  # the oracle library is not built on this host
+        oracle = {"skipped": str(ex)}
+    e32.destroy()
+    e64.destroy()
+    return dict(boxes={k: kinds.count(k) for k in dict.fromkeys(kinds)}, bulk="f32", layout=layout, steps=steps,
                 ms_per_ensemble_step=round(graph_ms, 3), box_steps_per_s=round(len(boxes) * 1000.0 / graph_ms, 1),
                 cell_updates_per_s=cells * 1000.0 / graph_ms,
-                eager_lanes_ms_per_step=round(eager_ms, 3), one_stream_ms_per_step=round(one_stream_ms, 3),
+                algorithmic_gb_per_ensemble_step=round(step_b / 1e9, 3),
+                achieved_gbs=round(step_b / (graph_ms * 1e-3) / 1e9, 1),
+                frac_of_hbm_peak=round(step_b / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                eager_batched_ms_per_step=round(eager_ms, 3), box_by_box_16_lanes_ms_per_step=None if box_ms is None else round(box_ms, 3),
                 rkl2_stages_hbi=[min(stages), max(stages)], setup_s=round(setup_s, 1),
-                accuracy_vs_f64=acc,
+                accuracy_f32_vs_f64=dict(steps=acc_steps, per_kind=acc), f64_vs_oracle=oracle,
                 params="nu, eta log-uniform [1e-4, 1e-2]; chi_par uniform [1e-3, 1e-1] (hbi); seeds 5000+i")
 
 
@@ -657,7 +720,7 @@ def main():
         print(json.dumps(mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu)))
         return
     if a.mode == "ensemble":
-        print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps)))
+        print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split)))
         return
     if a.mode == "closure-kernels":
         print(json.dumps(closure_kernels(spk, ctx, torch)))
@@ -694,6 +757,12 @@ def main():
     for k, v in kern.items():
         print("# kernel %-9s f64: %8.3f ms/launch x%d per step, %6.0f GB/s, %4.1f%% of step" %
               (k, v["ms"], v["per_step"], v["gbs"], 100 * v["share"]), file=sys.stderr, flush=True)
+    kern32 = None
+    if "f32" in res:
+        kern32 = kernel_table(spk, ctx, state, n, "f32", res["f32"]["fused"])
+        for k, v in kern32.items():
+            print("# kernel %-9s f32: %8.3f ms/launch x%d per step, %6.0f GB/s, %4.1f%% of step" %
+                  (k, v["ms"], v["per_step"], v["gbs"], 100 * v["share"]), file=sys.stderr, flush=True)
     dom_name = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["per_step"])
     dom = kern[dom_name]
     traffic = None
@@ -718,7 +787,7 @@ def main():
     mti = mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu) if a.extra else None
     if mti:
         print("# mti %d^3: %s" % (n, json.dumps(mti)), file=sys.stderr, flush=True)
-    ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps) if a.extra and a.ensemble_steps > 0 else None
+    ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split) if a.extra and a.ensemble_steps > 0 else None
 
     cpu = None
     acc_orc = None
@@ -762,6 +831,8 @@ def main():
         "fft_pass_gbs_f64": {k: round(v["gbs"], 1) for k, v in passes.items()},
         "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
                         for k, v in kern.items()},
+        "kernels_f32": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
+                        for k, v in kern32.items()} if kern32 else None,
         "roofline": roofline,
         "accuracy_vs_oracle": acc_orc,
         "cpu_baseline": cpu,

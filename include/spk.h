@@ -285,8 +285,13 @@ int spk_spectrum_ptr(spk_solver s, const void *complex_a, const void *complex_b,
  * The reference runs one box per process (Supervisor, supervisor.cu:20-213);
  * an ensemble holds nboxes solvers (box i: params[i], any grid and physics the
  * solver supports except shearing boxes, all in precision bulk) on device
- * `device`, spread over `lanes` HIP streams (box i on lane i % lanes; 0: 4).
- * Boxes with super-time-stepping plan their RKL2 cycles on the device. */
+ * `device`.  Boxes that differ only in nu, nu_m, nu_th and reynolds_ani form a
+ * shape group, and a group is advanced as one batch: each kernel of the step
+ * is launched once for all of its boxes (the box in blockIdx.z), so 48 boxes
+ * at 64^3 are one launch per kernel.  Batches run on their own HIP streams.
+ * `lanes` (0: 4) is the stream count of the box-by-box mode (split 0, box i on
+ * lane i % lanes).  Boxes with super-time-stepping plan their RKL2 cycles on
+ * the device, each with its own stage count. */
 typedef struct spk_ensemble_s *spk_ensemble;
 int spk_ensemble_create(int device, int nboxes, const spk_params *params, int bulk, int lanes, spk_ensemble *out);
 int spk_ensemble_destroy(spk_ensemble e);
@@ -302,11 +307,18 @@ int spk_ensemble_run(spk_ensemble e, int nsteps);
 int spk_ensemble_sync(spk_ensemble e);
 /* on = 0: the same steps launched box by box into the lanes, without a graph */
 int spk_ensemble_set_graph(spk_ensemble e, int on);
+/* box-by-box launches over `lanes` streams (split 0) */
 int spk_ensemble_set_lanes(spk_ensemble e, int lanes);
-/* Lane-count autotune over 1, 2, 4, 8, 16 lanes: each candidate times `probe`
- * (even) steps of every box after a warm-up pair; every box's state and clock
- * are saved before and restored after each candidate.  Keeps the fastest. */
-int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, double *best_ms);
+/* split >= 1: each shape group advanced in `split` batches of consecutive boxes
+ * (the default is 1: one batch per group); 0: box by box over the lanes */
+int spk_ensemble_set_split(spk_ensemble e, int split);
+int spk_ensemble_layout(spk_ensemble e, int *ngroups, int *split, int *nbatches);
+/* Autotune of the launch layout: batched, over splits 1, 2, 4; box by box,
+ * over 1, 2, 4, 8, 16 lanes.  Each candidate times `probe` (even) steps of
+ * every box after a warm-up pair; every box's state and clock are saved before
+ * and restored after each candidate.  Keeps the fastest; *best = its split
+ * (batched) or lane count. */
+int spk_ensemble_tune(spk_ensemble e, int probe, int *best, double *best_ms);
 
 /* ---- learned subgrid-stress closure (no counterpart in the reference) ----
  * A 4-layer 3D convolution network on a periodic n^3 grid (n a multiple of 32):

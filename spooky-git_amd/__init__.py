@@ -120,6 +120,8 @@ SIGNATURES = {
     "spk_ensemble_set_graph": [_VP, _I],
     "spk_ensemble_set_lanes": [_VP, _I],
     "spk_ensemble_tune": [_VP, _I, _VP, _VP],
+    "spk_ensemble_set_split": [_VP, _I],
+    "spk_ensemble_layout": [_VP, _VP, _VP, _VP],
     "spk_closure_num_params": [],
     "spk_closure_create": [_VP, _I, _VP],
     "spk_closure_destroy": [_VP],
@@ -467,8 +469,9 @@ class Solver:
 
 class Ensemble:
     """Many independent boxes advanced together (spk_ensemble): one solver per
-    box, boxes spread over HIP streams, two steps of every box per graph launch.
-    boxes: a list of parameter dicts (as Solver's keyword arguments)."""
+    box; boxes of one shape advanced as a batch (one launch per kernel for all
+    of them), batches on their own HIP streams, two steps of every box per graph
+    launch.  boxes: a list of parameter dicts (as Solver's keyword arguments)."""
 
     def __init__(self, boxes, precision="f32", lanes=0, device=0):
         self.params = [make_params(**b) for b in boxes]
@@ -503,9 +506,20 @@ class Ensemble:
         check(lib().spk_ensemble_set_graph(self.h, 1 if on else 0))
 
     def set_lanes(self, lanes):
+        """box-by-box launches over `lanes` streams (leaves batching)"""
         check(lib().spk_ensemble_set_lanes(self.h, lanes))
 
+    def set_split(self, split):
+        """split >= 1: each shape group advanced in `split` batches; 0: box by box"""
+        check(lib().spk_ensemble_set_split(self.h, split))
+
+    def layout(self):
+        g, s, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().spk_ensemble_layout(self.h, ctypes.byref(g), ctypes.byref(s), ctypes.byref(b)))
+        return dict(groups=g.value, split=s.value, batches=b.value, lanes=self.lanes())
+
     def tune(self, probe=2):
+        """batched: the best split of 1, 2, 4; box by box: the best lane count"""
         best, ms = ctypes.c_int(), ctypes.c_double()
         check(lib().spk_ensemble_tune(self.h, probe, ctypes.byref(best), ctypes.byref(ms)))
         return best.value, ms.value

@@ -23,6 +23,28 @@ void ctx_release(spk_ctx ctx) {
     if (ctx && ctx->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete ctx;
 }
 
+int make_batch(spk_ctx ctx, std::initializer_list<const void *> ptrs, Batch *out) {
+    memset(out, 0, sizeof(*out));
+    for (int i = 0; i < BATCH_ARGS; i++) out->slot[i] = -1;
+    if (!ctx->rows) return SPK_OK;
+    SPK_REQUIRE((int)ptrs.size() <= BATCH_ARGS, SPK_ERR_STATE, "batched launch: %zu pointer arguments", ptrs.size());
+    out->rows = ctx->rows;
+    int i = 0;
+    for (const void *p : ptrs) {
+        if (p != nullptr) {
+            const char *c = static_cast<const char *>(p);
+            int slot = -1;
+            for (int k = 0; k < BOX_SLOTS && slot < 0; k++)
+                if (ctx->base[k] && c >= ctx->base[k] && c < ctx->base[k] + ctx->size[k]) slot = k;
+            SPK_REQUIRE(slot >= 0, SPK_ERR_STATE, "batched launch: argument %d (%p) is in no array of the batch", i, p);
+            out->slot[i] = (signed char)slot;
+            out->off[i] = (long long)(c - ctx->base[slot]);
+        }
+        i++;
+    }
+    return SPK_OK;
+}
+
 }  // namespace spk
 
 extern "C" int spk_abi_version(void) { return SPK_ABI_VERSION; }

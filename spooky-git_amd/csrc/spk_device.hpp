@@ -364,4 +364,34 @@ __host__ __device__ constexpr int zpass_e(int m) { return line_e(m, 16); }
 __host__ __device__ constexpr int zpass_tpl(int m) { return m / zpass_e(m); }
 __host__ __device__ constexpr int zpass_rw(int m) { return zpass_tpl(m) >= 256 ? 1 : 256 / zpass_tpl(m); }
 
+
+// ---- batched launches over boxes of one shape ----
+// A batch is a set of boxes that share grid, physics and precision but own
+// their arrays and may differ in their diffusion coefficients.  A batched
+// launch runs one kernel over every box: blockIdx.z is the box, and each
+// pointer argument the launch lists is re-based onto that box's array of the
+// same kind (slot) at the same byte offset.  rows == nullptr is a plain
+// single-box launch and leaves every argument as passed.
+constexpr int BOX_SLOTS = 16;
+struct BoxRow {
+    char *p[BOX_SLOTS];                   // the box's arrays, by slot
+    double nu, nu_m, nu_th, reynolds_ani; // the coefficients a box may set for itself
+};
+constexpr int BATCH_ARGS = 8;
+struct Batch {
+    const BoxRow *rows;
+    signed char slot[BATCH_ARGS];  // -1: the argument is the same for every box
+    long long off[BATCH_ARGS];
+};
+
+// argument i of a batched launch for this workgroup's box
+template <class P>
+__device__ __forceinline__ P rebase(const Batch &b, int i, P p) {
+    if (b.rows == nullptr || b.slot[i] < 0) return p;
+    return reinterpret_cast<P>(b.rows[blockIdx.z].p[b.slot[i]] + b.off[i]);
+}
+__device__ __forceinline__ const BoxRow *batch_row(const Batch &b) {
+    return b.rows == nullptr ? nullptr : b.rows + blockIdx.z;
+}
+
 }  // namespace spk

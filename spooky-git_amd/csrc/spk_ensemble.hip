@@ -2,25 +2,51 @@
 //
 // Each box is a solver of its own (grid, physics, coefficients, precision of
 // the bulk, its own dt), so an ensemble of uneven boxes (64^3 and 128^3 MHD,
-// 512x512x2 Boussinesq-MHD with RKL2 conduction) is a list of solvers.  The
-// boxes are spread over HIP streams ("lanes"): a 64^3 transform fills a
-// fraction of the 256 CUs, so several boxes must run at once to fill the chip.
+// 512x512x2 Boussinesq-MHD with RKL2 conduction) is a list of solvers.
+//
+// Boxes of one shape -- the same grid, physics, precision and back end, with
+// diffusion coefficients (nu, nu_m, nu_th, chi) free per box -- form a group,
+// and a group is advanced as one batch: every kernel of its step is launched
+// once for all of its boxes, with the box in blockIdx.z, each box's arrays
+// looked up in a device table (BoxRow) and its coefficients overriding the
+// launch's.  A 64^3 transform fills a fraction of the 256 CUs; 48 of them in
+// one launch fill the chip.  A group may be cut into several batches
+// ("split"), each on its own HIP stream ("lane"), so that batches of different
+// groups and halves of one group overlap.  Split 0 launches box by box over
+// the lanes, as one solver each.
+//
 // Two steps of every box are captured once into one HIP graph (fork from an
 // origin stream into the lanes, join back) and each replay is a single graph
 // launch.  A captured step needs no host synchronization: dt stays on the
-// device and RKL2 cycles are planned on the device (spk_solver_set_rkl_device).
+// device and RKL2 cycles are planned on the device (spk_solver_set_rkl_device),
+// every box with its own stage count (stages past a box's count return at once).
 #include "spk_internal.hpp"
 
 #include <chrono>
 #include <vector>
+
+namespace {
+
+struct BoxBatch {
+    std::vector<int> boxes;         // indices into spk_ensemble_s::box; boxes[0] leads
+    int lane = 0;
+    spk::BoxRow *rows = nullptr;    // device table, one row per box
+    const char *base[spk::BOX_SLOTS] = {};  // the leader's arrays
+    size_t size[spk::BOX_SLOTS] = {};
+};
+
+}  // namespace
 
 struct spk_ensemble_s {
     int device = 0;
     int bulk = SPK_F64;
     int nlanes = 0;
     int use_graph = 1;
+    int split = 1;                    // 0: box by box; k >= 1: each group in k batches
     std::vector<spk_solver> box;
     std::vector<spk_ctx> bctx;        // one context per box; its stream is the box's lane
+    std::vector<std::vector<int>> groups;
+    std::vector<BoxBatch> batches;
     std::vector<hipStream_t> lanes;   // MAX_LANES streams, the first nlanes in use
     std::vector<hipEvent_t> ev_join;  // one per lane
     hipStream_t origin = nullptr;     // capture origin and graph launch stream
@@ -43,6 +69,12 @@ void drop_graph(spk_ensemble e) {
     e->graph = nullptr;
 }
 
+void drop_batches(spk_ensemble e) {
+    for (BoxBatch &b : e->batches)
+        if (b.rows) (void)hipFree(b.rows);
+    e->batches.clear();
+}
+
 // origin waits for every lane (work issued on a box's own stream before a
 // replay), or every lane waits for origin (calls on a box after a replay)
 int join_lanes_into_origin(spk_ensemble e) {
@@ -58,9 +90,39 @@ int fork_origin_into_lanes(spk_ensemble e) {
     return SPK_OK;
 }
 
-// Two steps of every box, boxes interleaved over the lanes (box i on lane i % nlanes).
+// Two steps of the boxes of one batch: one launch per kernel for all of them,
+// through the leader's context.  Boxes whose host step differs from the
+// leader's (one was stepped on its own) run box by box instead.
+int batch_two_steps(spk_ensemble e, const BoxBatch &b) {
+    spk_solver lead = e->box[b.boxes[0]];
+    bool aligned = true;
+    for (int i : b.boxes) aligned = aligned && solver_host_step(e->box[i]) == solver_host_step(lead);
+    if (!aligned || b.boxes.size() == 1) {
+        for (int i : b.boxes) SPK_TRY(spk_main_loop(e->box[i], 2));
+        return SPK_OK;
+    }
+    spk_ctx c = solver_ctx(lead);
+    c->rows = b.rows;
+    c->nbox = (int)b.boxes.size();
+    for (int k = 0; k < BOX_SLOTS; k++) {
+        c->base[k] = b.base[k];
+        c->size[k] = b.size[k];
+    }
+    const int rc = spk_main_loop(lead, 2);
+    c->rows = nullptr;
+    c->nbox = 1;
+    SPK_TRY(rc);
+    for (size_t k = 1; k < b.boxes.size(); k++) solver_advance_host(e->box[b.boxes[k]], 2);
+    return SPK_OK;
+}
+
+// Two steps of every box: batch by batch, or box by box over the lanes.
 int issue_two_steps(spk_ensemble e) {
-    for (spk_solver s : e->box) SPK_TRY(spk_main_loop(s, 2));
+    if (e->split == 0) {
+        for (spk_solver s : e->box) SPK_TRY(spk_main_loop(s, 2));
+        return SPK_OK;
+    }
+    for (const BoxBatch &b : e->batches) SPK_TRY(batch_two_steps(e, b));
     return SPK_OK;
 }
 
@@ -96,15 +158,60 @@ int capture(spk_ensemble e) {
     return SPK_OK;
 }
 
+int sync_all(spk_ensemble e) {
+    for (hipStream_t st : e->lanes) SPK_HIP(hipStreamSynchronize(st));
+    SPK_HIP(hipStreamSynchronize(e->origin));
+    return SPK_OK;
+}
+
+// Box by box over nlanes streams (box i on lane i % nlanes).
 int assign_lanes(spk_ensemble e, int nlanes) {
     SPK_REQUIRE(nlanes >= 1 && nlanes <= MAX_LANES, SPK_ERR_ARG, "spk_ensemble: %d lanes (1..%d)", nlanes,
                 MAX_LANES);
-    // finish what is queued before boxes change streams
-    for (int l = 0; l < MAX_LANES; l++) SPK_HIP(hipStreamSynchronize(e->lanes[l]));
-    SPK_HIP(hipStreamSynchronize(e->origin));
+    SPK_TRY(sync_all(e));
     drop_graph(e);
+    drop_batches(e);
+    e->split = 0;
     e->nlanes = nlanes;
     for (size_t i = 0; i < e->box.size(); i++) e->bctx[i]->stream = e->lanes[i % nlanes];
+    return SPK_OK;
+}
+
+// Each group cut into `split` batches of consecutive boxes; batch b on lane
+// b % MAX_LANES, and every box's own calls on its batch's lane.
+int assign_batches(spk_ensemble e, int split) {
+    SPK_REQUIRE(split >= 1, SPK_ERR_ARG, "spk_ensemble: split %d (at least 1)", split);
+    SPK_TRY(sync_all(e));
+    drop_graph(e);
+    drop_batches(e);
+    e->split = split;
+    for (const std::vector<int> &g : e->groups) {
+        const int n = (int)g.size();
+        const int k = split < n ? split : n;
+        for (int j = 0; j < k; j++) {
+            BoxBatch b;
+            for (int i = (int)((long long)n * j / k); i < (int)((long long)n * (j + 1) / k); i++) b.boxes.push_back(g[i]);
+            b.lane = (int)e->batches.size() % MAX_LANES;
+            e->batches.push_back(b);
+        }
+    }
+    e->nlanes = (int)e->batches.size() < MAX_LANES ? (int)e->batches.size() : MAX_LANES;
+    for (BoxBatch &b : e->batches) {
+        std::vector<BoxRow> h(b.boxes.size());
+        for (size_t k = 0; k < b.boxes.size(); k++) {
+            const int i = b.boxes[k];
+            size_t sz[BOX_SLOTS];
+            solver_box_row(e->box[i], &h[k], sz);
+            if (k == 0)
+                for (int q = 0; q < BOX_SLOTS; q++) {
+                    b.base[q] = h[0].p[q];
+                    b.size[q] = sz[q];
+                }
+            e->bctx[i]->stream = e->lanes[b.lane];
+        }
+        SPK_HIP(hipMalloc((void **)&b.rows, sizeof(BoxRow) * h.size()));
+        SPK_HIP(hipMemcpy(b.rows, h.data(), sizeof(BoxRow) * h.size(), hipMemcpyHostToDevice));
+    }
     return SPK_OK;
 }
 
@@ -112,6 +219,7 @@ void destroy_all(spk_ensemble e) {
     for (hipStream_t st : e->lanes) (void)hipStreamSynchronize(st);
     if (e->origin) (void)hipStreamSynchronize(e->origin);
     drop_graph(e);
+    drop_batches(e);
     for (spk_solver s : e->box) spk_solver_destroy(s);
     for (spk_ctx c : e->bctx) spk_ctx_destroy(c);
     for (hipEvent_t ev : e->ev_join) (void)hipEventDestroy(ev);
@@ -145,9 +253,18 @@ int create_impl(spk_ensemble e, int device, int nboxes, const spk_params *params
         SPK_TRY(spk_solver_create(c, &p, bulk, &s));
         e->box.push_back(s);
         if (p.supertimestepping && p.sts_mask) SPK_TRY(spk_solver_set_rkl_device(s, 1));
+        // the first box of each shape leads its group; the others take its back end
+        int g = 0;
+        while (g < (int)e->groups.size() && !solver_same_shape(e->box[e->groups[g][0]], s)) g++;
+        if (g == (int)e->groups.size()) {
+            e->groups.push_back({i});
+        } else {
+            SPK_TRY(solver_follow_back_end(s, e->box[e->groups[g][0]]));
+            e->groups[g].push_back(i);
+        }
     }
     e->nlanes = nlanes;
-    return SPK_OK;
+    return assign_batches(e, 1);
 }
 
 }  // namespace
@@ -183,6 +300,14 @@ extern "C" int spk_ensemble_size(spk_ensemble e, int *nboxes, int *lanes) {
     return SPK_OK;
 }
 
+extern "C" int spk_ensemble_layout(spk_ensemble e, int *ngroups, int *split, int *nbatches) {
+    SPK_REQUIRE(e, SPK_ERR_ARG, "spk_ensemble_layout: null ensemble");
+    if (ngroups) *ngroups = (int)e->groups.size();
+    if (split) *split = e->split;
+    if (nbatches) *nbatches = (int)e->batches.size();
+    return SPK_OK;
+}
+
 extern "C" int spk_ensemble_box(spk_ensemble e, int i, spk_solver *out) {
     SPK_REQUIRE(e && out && i >= 0 && i < (int)e->box.size(), SPK_ERR_ARG, "spk_ensemble_box: bad arguments");
     *out = e->box[i];
@@ -199,6 +324,13 @@ extern "C" int spk_ensemble_set_graph(spk_ensemble e, int on) {
 extern "C" int spk_ensemble_set_lanes(spk_ensemble e, int lanes) {
     SPK_REQUIRE(e, SPK_ERR_ARG, "spk_ensemble_set_lanes: null ensemble");
     return assign_lanes(e, lanes);
+}
+
+extern "C" int spk_ensemble_set_split(spk_ensemble e, int split) {
+    SPK_REQUIRE(e, SPK_ERR_ARG, "spk_ensemble_set_split: null ensemble");
+    SPK_REQUIRE(split >= 0, SPK_ERR_ARG, "spk_ensemble_set_split: split %d", split);
+    if (split == 0) return assign_lanes(e, e->nlanes > 0 ? e->nlanes : 1);
+    return assign_batches(e, split);
 }
 
 extern "C" int spk_ensemble_run(spk_ensemble e, int nsteps) {
@@ -228,13 +360,15 @@ extern "C" int spk_ensemble_sync(spk_ensemble e) {
     return SPK_OK;
 }
 
-// Lane-count autotune: each candidate runs `probe` (even) steps of every box,
-// timed after one warm-up replay; the state and clock of every box are saved
-// before and restored after each candidate, so tuning leaves the boxes as it
-// found them.  The fastest count is kept.
-extern "C" int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, double *best_ms) {
+// Autotune of the launch layout: in batched mode (split >= 1) over splits 1, 2
+// and 4, box by box (split 0) over 1, 2, 4, 8 and 16 lanes.  Each candidate
+// runs `probe` (even) steps of every box, timed after one warm-up replay; the
+// state and clock of every box are saved before and restored after each
+// candidate, so tuning leaves the boxes as it found them.  The fastest is kept.
+extern "C" int spk_ensemble_tune(spk_ensemble e, int probe, int *best_out, double *best_ms) {
     SPK_REQUIRE(e && probe > 0 && probe % 2 == 0, SPK_ERR_ARG, "spk_ensemble_tune: bad arguments");
     SPK_TRY(spk_ensemble_sync(e));
+    const bool batched = e->split > 0;
     const int nb = (int)e->box.size();
     std::vector<void *> backup(nb, nullptr);
     std::vector<size_t> bytes(nb);
@@ -271,13 +405,13 @@ extern "C" int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, dou
         }
         return SPK_OK;
     };
-    int best = e->nlanes;
+    int best = batched ? e->split : e->nlanes;
     double tbest = 1e300;
-    const int cand[] = {1, 2, 4, 8, 16};
+    const std::vector<int> cand = batched ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2, 4, 8, 16};
     for (int L : cand) {
         if (rc != SPK_OK) break;
-        if (L > nb && L != 1) continue;
-        rc = assign_lanes(e, L);
+        if (!batched && L > nb && L != 1) continue;
+        rc = batched ? assign_batches(e, L) : assign_lanes(e, L);
         if (rc == SPK_OK) rc = spk_ensemble_run(e, 2);
         if (rc == SPK_OK) rc = spk_ensemble_sync(e);
         if (rc != SPK_OK) break;
@@ -291,11 +425,11 @@ extern "C" int spk_ensemble_tune(spk_ensemble e, int probe, int *best_lanes, dou
         }
         if (rc == SPK_OK) rc = restore();
     }
-    if (rc == SPK_OK) rc = assign_lanes(e, best);
+    if (rc == SPK_OK) rc = batched ? assign_batches(e, best) : assign_lanes(e, best);
     for (void *b : backup)
         if (b) (void)hipFree(b);
     if (rc != SPK_OK) return rc;
-    if (best_lanes) *best_lanes = best;
+    if (best_out) *best_out = best;
     if (best_ms) *best_ms = tbest;
     return SPK_OK;
 }

@@ -36,7 +36,10 @@ template <class T, int N, int DIR, int AXIS, int TC = strided_tcols(N)>
 __global__ __launch_bounds__(TC * strided_tpl(N)) void k_pass_strided(
     const cpx<T> *in, cpx<T> *out, long long ncols, long long lstride, long long ostride,
     int nouter, long long fs_in, long long fs_out, const cpx<T> *__restrict__ tw, T scale,
-    const int *__restrict__ gate) {
+    const int *__restrict__ gate, Batch bt) {
+    in = rebase(bt, 0, in);
+    out = rebase(bt, 1, out);
+    gate = rebase(bt, 2, gate);
     if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
     constexpr int TPL = N / E;
@@ -72,13 +75,15 @@ __global__ __launch_bounds__(TC * strided_tpl(N)) void k_pass_strided(
 template <class T, int NZ>
 __global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z_r2c(
     const T *in, cpx<T> *out, long long nrows, long long fs_in, long long fs_out,
-    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m) {
+    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m, Batch bt) {
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;
     constexpr int E = zpass_e(M);
     using Row = ZRow<T, M, E>;
     constexpr int RW = zpass_rw(M);
     __shared__ cpx<T> lds[RW * Row::LL];
+    in = rebase(bt, 0, in);
+    out = rebase(bt, 1, out);
 
     const int tid = threadIdx.x;
     const int rl = tid / Row::TPL, t = tid % Row::TPL;
@@ -98,13 +103,15 @@ __global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z
 template <class T, int NZ>
 __global__ __launch_bounds__(zpass_rw(NZ / 2) * zpass_tpl(NZ / 2)) void k_pass_z_c2r(
     const cpx<T> *in, T *out, long long nrows, long long fs_in, long long fs_out,
-    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m) {
+    const cpx<T> *__restrict__ tw_nz, const cpx<T> *__restrict__ tw_m, Batch bt) {
     constexpr int M = NZ / 2;
     constexpr int NZH = M + 1;
     constexpr int E = zpass_e(M);
     using Row = ZRow<T, M, E>;
     constexpr int RW = zpass_rw(M);
     __shared__ cpx<T> lds[RW * Row::LL];
+    in = rebase(bt, 0, in);
+    out = rebase(bt, 1, out);
 
     const int tid = threadIdx.x;
     const int rl = tid / Row::TPL, t = tid % Row::TPL;
@@ -180,23 +187,26 @@ static const cpx<T> *tw_ptr(const Twiddles &tw, long long off) {
 template <class T, int DIR, int AXIS>
 static int launch_strided(spk_ctx ctx, int n, long long ncols, long long lstride, long long ostride,
                           int nouter, const cpx<T> *twp, const PassArgs &a) {
+    Batch bt;
+    SPK_TRY(make_batch(ctx, {a.in, a.out, ctx->gate}, &bt));
+    const unsigned nbz = nboxes(ctx);
 #define SPK_STRIDED_CASE(NN)                                                                       \
     case NN: {                                                                                     \
         constexpr int TC = strided_tcols_t<T>(NN);                                                 \
         if constexpr (AXIS == 1 && TC > 2) {                                                       \
             if (ncols == 2) {                                                                      \
-                dim3 grid((unsigned)nouter, a.nfields);                                           \
+                dim3 grid((unsigned)nouter, a.nfields, nbz);                                      \
                 k_pass_strided<T, NN, DIR, AXIS, 2><<<grid, 2 * strided_tpl(NN), 0, ctx->stream>>>(   \
                     (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, nouter,         \
-                    a.fstride_in, a.fstride_out, twp, (T)a.scale, ctx->gate);                      \
+                    a.fstride_in, a.fstride_out, twp, (T)a.scale, ctx->gate, bt);                  \
                 break;                                                                             \
             }                                                                                      \
         }                                                                                          \
         const long long ntiles = (ncols * nouter + TC - 1) / TC;                                   \
-        dim3 grid((unsigned)ntiles, a.nfields);                                                    \
+        dim3 grid((unsigned)ntiles, a.nfields, nbz);                                               \
         k_pass_strided<T, NN, DIR, AXIS, TC><<<grid, TC * strided_tpl(NN), 0, ctx->stream>>>(         \
             (const cpx<T> *)a.in, (cpx<T> *)a.out, ncols, lstride, ostride, nouter, a.fstride_in, \
-            a.fstride_out, twp, (T)a.scale, ctx->gate);                                            \
+            a.fstride_out, twp, (T)a.scale, ctx->gate, bt);                                        \
         break;                                                                                     \
     }
     switch (n) {
@@ -241,19 +251,22 @@ static int launch_z(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, bool r2c, c
     const long long nrows = (long long)g.nx * g.ny;
     const cpx<T> *tz = tw_ptr<T>(tw, tw.off_z);
     const cpx<T> *tm = tw_ptr<T>(tw, tw.off_z + g.nz);
+    Batch bt;
+    SPK_TRY(make_batch(ctx, {a.in, a.out}, &bt));
+    const unsigned nbz = nboxes(ctx);
 #define SPK_Z_CASE(NN)                                                                         \
     case NN: {                                                                                 \
         constexpr int RW = zpass_rw(NN / 2);                                                   \
-        dim3 grid((unsigned)((nrows + RW - 1) / RW), a.nfields);                               \
+        dim3 grid((unsigned)((nrows + RW - 1) / RW), a.nfields, nbz);                          \
         const int bs = RW * zpass_tpl(NN / 2);                                                 \
         if (r2c)                                                                               \
             k_pass_z_r2c<T, NN><<<grid, bs, 0, ctx->stream>>>((const T *)a.in, (cpx<T> *)a.out, \
                                                               nrows, a.fstride_in,             \
-                                                              a.fstride_out, tz, tm);          \
+                                                              a.fstride_out, tz, tm, bt);      \
         else                                                                                   \
             k_pass_z_c2r<T, NN><<<grid, bs, 0, ctx->stream>>>((const cpx<T> *)a.in, (T *)a.out, \
                                                               nrows, a.fstride_in,             \
-                                                              a.fstride_out, tz, tm);          \
+                                                              a.fstride_out, tz, tm, bt);      \
         break;                                                                                 \
     }
     switch (g.nz) {

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <initializer_list>
 #include <string>
 
 #include "../../include/spk.h"
@@ -23,6 +24,14 @@ struct spk_ctx_s {
     // *gate is non-zero (device-sized loops: the stages of an RKL2 cycle beyond
     // the stage count the device computed for this step)
     const int *gate = nullptr;
+    // Batched launches (spk_ensemble): while rows is set, every kernel launched on
+    // this context runs over nbox boxes of one shape.  base/size are the arrays
+    // of the box whose addresses the launch sites pass, slot by slot, so that a
+    // pointer argument maps to its slot and byte offset (make_batch).
+    const spk::BoxRow *rows = nullptr;
+    int nbox = 1;
+    const char *base[spk::BOX_SLOTS] = {};
+    size_t size[spk::BOX_SLOTS] = {};
 };
 
 namespace spk {
@@ -33,6 +42,15 @@ void set_error(const char *fmt, ...);
 // context reference counting (spk_core.hip)
 spk_ctx ctx_retain(spk_ctx ctx);
 void ctx_release(spk_ctx ctx);
+
+// The Batch of a launch on ctx whose pointer arguments are ptrs, in the order
+// the kernel re-bases them: each non-null pointer is mapped to the slot of the
+// array it lies in and its byte offset there.  Outside a batched run the result
+// is a plain launch.  A pointer in none of the arrays is an error (the launch
+// would touch one box only).
+int make_batch(spk_ctx ctx, std::initializer_list<const void *> ptrs, Batch *out);
+// blockIdx.z extent of a launch on ctx
+inline unsigned nboxes(spk_ctx ctx) { return ctx->rows ? (unsigned)ctx->nbox : 1u; }
 
 // Evaluates a HIP call; on failure records file:line and the HIP message and
 // returns SPK_ERR_HIP from the enclosing function.
@@ -129,5 +147,19 @@ int noalloc_setup(int device);
 spk_ctx solver_ctx(spk_solver s);
 int solver_host_step(spk_solver s);
 void solver_advance_host(spk_solver s, int n);
+// batch slots of a solver's arrays
+enum : int {
+    SL_U = 0, SL_S, SL_DU, SL_REAL, SL_W, SL_P, SL_W2, SL_P2, SL_BREAL, SL_TS, SL_PMAX, SL_UC0, SL_UC1, SL_DU0,
+    SL_PLAN, SL_COUNT
+};
+static_assert(SL_COUNT <= BOX_SLOTS, "batch slots");
+// the row of s in a batch table (arrays by slot, coefficients) and the byte
+// size of each array
+void solver_box_row(spk_solver s, BoxRow *row, size_t *sizes);
+// a and b can run in one batch: same grid, physics, precision, back end and
+// every parameter but nu, nu_m, nu_th and reynolds_ani
+bool solver_same_shape(spk_solver a, spk_solver b);
+// s takes the back end (fused or not, points per thread) that lead runs
+int solver_follow_back_end(spk_solver s, spk_solver lead);
 
 }  // namespace spk

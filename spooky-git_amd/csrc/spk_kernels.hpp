@@ -50,6 +50,30 @@ __device__ __forceinline__ void kvec_at(const KGeom &g, int i, int j, int k, dou
 }
 
 // ------------------------------------------------------------------------
+// RKL2 stage (rkl.cu:496-585, rkl2_stage cuda_kernels_generic.cu:160-172) fused
+// with ParabolicTermsSTS (physics.cu:233-330) for every sts variable
+// ------------------------------------------------------------------------
+struct RklConst {
+    KGeom kg;
+    long long ntc;
+    int nsts;
+    int field[8];    // state index of sts variable n
+    int kind[8];     // 0: -nu k^2, 1: -nu_m k^2, 2: -nu_th k^2, 3: div of anisotropic flux
+    double nu, nu_m, nu_th;
+};
+
+// Device-sized RKL2 cycle: the stage count and coefficient table of the
+// recurrence (rkl.cu:466-585) computed on the device from the step's dt and
+// dt_sts, so that a cycle needs no host synchronization and can be captured in
+// a graph.  RKL_SMAX stages bound what a plan can hold.
+constexpr int RKL_SMAX = 64;
+struct RklPlan {
+    int s, overflow;
+    int active[RKL_SMAX + 1];   // active[j] = 1 for stages 1 <= j <= s
+    double coef[4 * RKL_SMAX];  // mu, nu, mu~, gamma~ of stage j at 4 (j - 1)
+};
+
+// ------------------------------------------------------------------------
 // inverse x pass from the FP64 state, with optional i*k_d derivative
 // (Gradient, cuda_kernels_generic.cu:249-260) and the 1/N scaling of c2r_fft
 // ------------------------------------------------------------------------
@@ -65,7 +89,10 @@ struct InvProg {
 template <class T, int N>
 __global__ __launch_bounds__(strided_tcols(N) * strided_tpl(N)) void k_inv_x(
     const cpx<double> *__restrict__ U, cpx<T> *__restrict__ W, long long fs, InvProg prog, KGeom kg,
-    double scale, const cpx<T> *__restrict__ tw, const int *__restrict__ gate) {
+    double scale, const cpx<T> *__restrict__ tw, const int *__restrict__ gate, Batch bt) {
+    U = rebase(bt, 0, U);
+    W = rebase(bt, 1, W);
+    gate = rebase(bt, 2, gate);
     if (gate != nullptr && *gate == 0) return;
     constexpr int E = strided_e(N);
     constexpr int TPL = N / E;
@@ -208,7 +235,13 @@ __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, cons
                                               cpx<T> *__restrict__ P, long long nrows, long long fs,
                                               const cpx<T> *__restrict__ tw_nz,
                                               const cpx<T> *__restrict__ tw_m, ZConst zc,
-                                              double *__restrict__ pmax, const int *__restrict__ gate) {
+                                              double *__restrict__ pmax, const int *__restrict__ gate, Batch bt) {
+    W = rebase(bt, 0, W);
+    Wr = rebase(bt, 1, Wr);
+    P = rebase(bt, 2, P);
+    pmax = rebase(bt, 3, pmax);
+    gate = rebase(bt, 4, gate);
+    if (const BoxRow *br = batch_row(bt)) zc.chi = 1.0 / br->reynolds_ani;
     if (gate != nullptr && *gate == 0) return;
     using Z = ZProg<PROG>;
     constexpr int M = NZ / 2;
@@ -349,8 +382,18 @@ __device__ __forceinline__ cpx<double> diffuse(int mask, int q, cpx<double> d, c
 template <class T, int LY, int MODE>
 __global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, cpx<double> *__restrict__ U,
                                                   cpx<double> *__restrict__ S, cpx<double> *__restrict__ dU,
-                                                  EpiConst c, const TimeState *__restrict__ ts) {
+                                                  EpiConst c, const TimeState *__restrict__ ts, Batch bt) {
     using L = Layout<LY>;
+    P = rebase(bt, 0, P);
+    U = rebase(bt, 1, U);
+    S = rebase(bt, 2, S);
+    dU = rebase(bt, 3, dU);
+    ts = rebase(bt, 4, ts);
+    if (const BoxRow *br = batch_row(bt)) {
+        c.nu = br->nu;
+        c.nu_m = br->nu_m;
+        c.nu_th = br->nu_th;
+    }
     constexpr int NF = L::NF, VX = L::VX, BX = L::BX, TH = L::TH;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= c.ntc) return;
@@ -487,7 +530,11 @@ __global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, 
 // tile autotune); TPL = N/EF threads per line, TCOLS columns per 256-thread block
 __host__ __device__ constexpr int fx_e(int n, int ef) { return line_e(n, ef); }
 __host__ __device__ constexpr int fx_tpl(int n, int ef) { return n / fx_e(n, ef); }
-__host__ __device__ constexpr int fx_tcols(int n, int ef) { return fx_tpl(n, ef) >= 512 ? 1 : 512 / fx_tpl(n, ef); }
+// at most 64 columns: a lane past the last column repeats the last column's
+// loads and stores, which is safe only within the wave that owns that column
+__host__ __device__ constexpr int fx_tcols(int n, int ef) {
+    return fx_tpl(n, ef) >= 512 ? 1 : (512 / fx_tpl(n, ef) < 64 ? 512 / fx_tpl(n, ef) : 64);
+}
 
 // RK3 registers of the fused path.  Wray's scheme (hydro_mhd_advance.cu:114-164)
 //   U1 = U0 + g1 dt L0, S1 = U1 + x1 dt L0, U2 = S1 + g2 dt L1, S2 = U2 + x2 dt L1,
@@ -750,8 +797,18 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 template <class T, int N, int LY, int MODE, int EF>
 __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
     const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ S, cpx<T> *__restrict__ W,
-    FusedConst fc, const cpx<T> *__restrict__ tw, const TimeState *__restrict__ ts) {
+    FusedConst fc, const cpx<T> *__restrict__ tw, const TimeState *__restrict__ ts, Batch bt) {
     using L = Layout<LY>;
+    P = rebase(bt, 0, P);
+    U = rebase(bt, 1, U);
+    S = rebase(bt, 2, S);
+    W = rebase(bt, 3, W);
+    ts = rebase(bt, 4, ts);
+    if (const BoxRow *br = batch_row(bt)) {
+        fc.e.nu = br->nu;
+        fc.e.nu_m = br->nu_m;
+        fc.e.nu_th = br->nu_th;
+    }
     constexpr int E = fx_e(N, EF);
     constexpr int TPL = N / E;
     constexpr int TC = fx_tcols(N, EF);

@@ -1,0 +1,183 @@
+// spk_rklfused.hip -- one RKL2 stage of anisotropic conduction as a single
+// sweep: forward x pass of the heat-flux products + flux divergence + the RKL2
+// stage update + the next stage's inverse x pass of i k theta.
+//
+// An RKL2 stage of the anisotropic conduction of theta (rkl.cu:496-585 with
+// ParabolicTermsSTS, physics.cu:233-330, and AnisotropicHeatFlux,
+// anisotropic_conduction.cu:10-62) is, per stage: i k theta to real space, the
+// flux chi b (b . grad theta + OmegaT2 b_s) / B^2 there, its forward transform,
+// i mask k . flux, and the rkl2_stage update (cuda_kernels_generic.cu:160-172).
+// The unfused cycle runs the forward x pass, the update and the next stage's
+// inverse x pass as three launches with the flux and i k theta through HBM in
+// between.  Here one block owns TC columns of whole x lines: it transforms the
+// three flux products along x in registers and LDS, applies the update point by
+// point, and transforms i k_d theta of the new theta back along x into the next
+// stage's input slots.  The next stage starts at its inverse y pass.
+#include "spk_internal.hpp"
+#include "spk_kernels.hpp"
+
+namespace spk {
+
+template <class T, int N>
+__global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
+    const cpx<T> *Q, cpx<double> *__restrict__ U, cpx<double> *__restrict__ Uc0, cpx<double> *__restrict__ Uc1,
+    cpx<double> *__restrict__ dU0, cpx<T> *__restrict__ W, RklConst c, int stage, double mu, double nu,
+    double dtmut, double gdt, int inv, double scale, const cpx<T> *__restrict__ tw,
+    const RklPlan *__restrict__ plan, const TimeState *__restrict__ ts, Batch bt) {
+    Q = rebase(bt, 0, Q);
+    U = rebase(bt, 1, U);
+    Uc0 = rebase(bt, 2, Uc0);
+    Uc1 = rebase(bt, 3, Uc1);
+    dU0 = rebase(bt, 4, dU0);
+    W = rebase(bt, 5, W);
+    plan = rebase(bt, 6, plan);
+    ts = rebase(bt, 7, ts);
+    if (plan != nullptr) {
+        // device-planned cycle: stages past this box's count return at once; the
+        // last active stage writes no next-stage input
+        if (!plan->active[stage]) return;
+        const double *cf = plan->coef + 4 * (stage - 1);
+        const double dt = ts->dt;
+        mu = cf[0];
+        nu = cf[1];
+        dtmut = cf[2] * dt;
+        gdt = cf[3] * dt;
+        inv = inv && plan->active[stage + 1];
+    }
+    constexpr int E = fx_e(N, 8);
+    constexpr int TPL = N / E;
+    constexpr int TC = fx_tcols(N, 8);
+    constexpr int LL = lds_len(N);
+    constexpr int NT = TC * TPL;
+    __shared__ cpx<T> lds[E != N ? TC * LL : 1];
+    __shared__ cpx<T> twl[N];
+    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    const int tid = threadIdx.x;
+    const int cl = tid % TC, t = tid / TC;
+    const long long col0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * TC + cl;
+    // a lane past the last column transforms the last column (every lane takes
+    // part in the transforms' barriers) and stores nothing
+    const bool live = col0 < ncols;
+    const long long col = live ? col0 : ncols - 1;
+    cpx<T> *ldsc = lds + cl * LL;
+    const long long N0 = c.ntc;
+    const int j = (int)(col / c.kg.nzh);
+    const int kk = (int)(col - (long long)j * c.kg.nzh);
+    const long long F = (long long)c.field[0] * N0;
+
+    for (int e = tid; e < N; e += NT) twl[e] = tw[e];
+    __syncthreads();
+
+    // the three flux components along x, the next one loaded while one is transformed
+    cpx<double> acc[3][E];
+    cpx<T> vn[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) vn[i] = Q[col + (long long)(t + i * TPL) * ncols];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = vn[i];
+        if (q + 1 < 3) {
+#pragma unroll
+            for (int i = 0; i < E; i++) vn[i] = Q[(q + 1) * N0 + col + (long long)(t + i * TPL) * ncols];
+        }
+        LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
+#pragma unroll
+        for (int i = 0; i < E; i++) acc[q][i] = cconv<double>(v[i]);
+    }
+
+    // per point: dtheta = i mask k . flux (DivergenceMask), then rkl2_stage
+    const double w = 1.0 - mu - nu;
+    cpx<double> th[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = t + i * TPL;
+        const long long pi = col + (long long)n * ncols;
+        double kx, ky, kz, m;
+        kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
+        const cpx<double> du = mul_plus_im(m, kdot3(kx, acc[0][i], ky, acc[1][i], kz, acc[2][i]));
+        const cpx<double> uc = U[F + pi];
+        cpx<double> un;
+        if (stage == 1) {
+            if (live) {
+                Uc0[pi] = uc;
+                dU0[pi] = du;
+                Uc1[pi] = uc;
+            }
+            un = cpx<double>{uc.x + dtmut * du.x, uc.y + dtmut * du.y};
+        } else {
+            const cpx<double> u1 = Uc1[pi], u0 = Uc0[pi], d0 = dU0[pi];
+            const cpx<double> Y = {mu * uc.x + nu * u1.x, mu * uc.y + nu * u1.y};
+            if (live) Uc1[pi] = uc;
+            un = cpx<double>{Y.x + w * u0.x + dtmut * du.x + gdt * d0.x, Y.y + w * u0.y + dtmut * du.y + gdt * d0.y};
+        }
+        if (live) U[F + pi] = un;
+        th[i] = un;
+    }
+
+    if (inv) {
+        // i k_d theta of the new theta, 1/N scaled, inverse x pass (k_inv_x with
+        // the derivative slots of the RKL2 program) into the next stage's slots
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            cpx<T> v[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                const int n = t + i * TPL;
+                double kx, ky, kz, m;
+                kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
+                const double kd = d == 0 ? kx : (d == 1 ? ky : kz);
+                const cpx<double> a = {-(kd * th[i].y), kd * th[i].x};
+                v[i] = cconv<T>(cscale(a, scale));
+            }
+            LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
+            if (live) {
+                cpx<T> *dst = W + d * N0 + col;
+#pragma unroll
+                for (int i = 0; i < E; i++) dst[(long long)(t + i * TPL) * ncols] = v[i];
+            }
+        }
+    }
+}
+
+bool rkl_fused_supported_nx(int nx) { return nx >= 32 && nx <= 1024 && is_pow2(nx); }
+
+template <class T>
+static int launch_t(spk_ctx ctx, int nx, const void *Q, void *U, void *Uc0, void *Uc1, void *dU0, void *W,
+                    const RklConst &c, int stage, double mu, double nu, double dtmut, double gdt, int inv, double scale,
+                    const void *tw, const RklPlan *plan, const TimeState *ts) {
+    Batch bt;
+    SPK_TRY(make_batch(ctx, {Q, U, Uc0, Uc1, dU0, W, plan, ts}, &bt));
+    const long long ncols = (long long)c.kg.ny * c.kg.nzh;
+    const cpx<T> *q = (const cpx<T> *)Q;
+    cpx<double> *u = (cpx<double> *)U, *u0 = (cpx<double> *)Uc0, *u1 = (cpx<double> *)Uc1, *d0 = (cpx<double> *)dU0;
+    cpx<T> *w = (cpx<T> *)W;
+    const cpx<T> *t = (const cpx<T> *)tw;
+#define SPK_RKLF_CASE(NN)                                                                                      \
+    case NN: {                                                                                                 \
+        constexpr int TC = fx_tcols(NN, 8);                                                                    \
+        const dim3 grid((unsigned)((ncols + TC - 1) / TC), 1, nboxes(ctx));                                    \
+        k_fwdx_rkl2<T, NN><<<grid, TC * fx_tpl(NN, 8), 0, ctx->stream>>>(q, u, u0, u1, d0, w, c, stage, mu, nu, \
+                                                                          dtmut, gdt, inv, scale, t, plan, ts, bt); \
+        break;                                                                                                 \
+    }
+    switch (nx) {
+        SPK_RKLF_CASE(32) SPK_RKLF_CASE(64) SPK_RKLF_CASE(128) SPK_RKLF_CASE(256) SPK_RKLF_CASE(512)
+        SPK_RKLF_CASE(1024)
+        default: set_error("fused RKL2 stage: NX=%d not instantiated", nx); return SPK_ERR_ARG;
+    }
+#undef SPK_RKLF_CASE
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
+int launch_fwdx_rkl2(spk_ctx ctx, int bulk, int nx, const void *Q, void *U, void *Uc0, void *Uc1, void *dU0, void *W,
+                     const RklConst &c, int stage, double mu, double nu, double dtmut, double gdt, int inv,
+                     double scale, const void *tw, const RklPlan *plan, const TimeState *ts) {
+    if (bulk == SPK_F64)
+        return launch_t<double>(ctx, nx, Q, U, Uc0, Uc1, dU0, W, c, stage, mu, nu, dtmut, gdt, inv, scale, tw, plan, ts);
+    return launch_t<float>(ctx, nx, Q, U, Uc0, Uc1, dU0, W, c, stage, mu, nu, dtmut, gdt, inv, scale, tw, plan, ts);
+}
+
+}  // namespace spk

@@ -35,8 +35,16 @@ struct DtConst {
     int nblocks;
 };
 
-__global__ __launch_bounds__(256) void k_dt(const double *__restrict__ pmax, DtConst c, TimeState *ts) {
+__global__ __launch_bounds__(256) void k_dt(const double *__restrict__ pmax, DtConst c, TimeState *ts, Batch bt) {
     __shared__ double sm[6][256];
+    pmax = rebase(bt, 0, pmax);
+    ts = rebase(bt, 1, ts);
+    if (const BoxRow *br = batch_row(bt)) {
+        c.nu = br->nu;
+        c.nu_m = br->nu_m;
+        c.nu_th = br->nu_th;
+        c.reynolds_ani = br->reynolds_ani;
+    }
     const int tid = threadIdx.x;
     double mx[6] = {0, 0, 0, 0, 0, 0};
     if (c.incompressible) {
@@ -122,8 +130,9 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ pmax, DtC
 // clock updates: t += dt (RungeKutta3), step++ (HydroMHDAdvance), snapshot
 // clock of CheckSnapshot (check_output.cu:67-83), tremap from the host
 __global__ void k_clock(TimeState *ts, int add_dt, int add_step, int snap, double toutput_flow,
-                        int set_tremap, double tremap) {
+                        int set_tremap, double tremap, Batch bt) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ts = rebase(bt, 0, ts);
     if (add_dt) ts->t += ts->dt;
     if (add_step) ts->step += 1.0;
     if (set_tremap) ts->tremap = tremap;
@@ -142,7 +151,8 @@ __global__ void k_set_time(TimeState *ts, double t, double step, double tls, dou
 // CleanDivergence (cuda_kernels_generic.cu:288-313), in place on u and B
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_clean_div(cpx<double> *__restrict__ U, long long ntc, KGeom kg,
-                                                   int f0, int f1) {
+                                                   int f0, int f1, Batch bt) {
+    U = rebase(bt, 0, U);
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ntc) return;
     const long long rowi = idx / kg.nzh;
@@ -169,35 +179,13 @@ __global__ __launch_bounds__(256) void k_clean_div(cpx<double> *__restrict__ U, 
     }
 }
 
-// ------------------------------------------------------------------------
-// RKL2 stage (rkl.cu:496-585, rkl2_stage cuda_kernels_generic.cu:160-172) fused
-// with ParabolicTermsSTS (physics.cu:233-330) for every sts variable
-// ------------------------------------------------------------------------
-struct RklConst {
-    KGeom kg;
-    long long ntc;
-    int nsts;
-    int field[8];    // state index of sts variable n
-    int kind[8];     // 0: -nu k^2, 1: -nu_m k^2, 2: -nu_th k^2, 3: div of anisotropic flux
-    double nu, nu_m, nu_th;
-};
-
-// Device-sized RKL2 cycle: the stage count and coefficient table of the
-// recurrence (rkl.cu:466-585) computed on the device from the step's dt and
-// dt_sts, so that a cycle needs no host synchronization and can be captured in
-// a graph.  RKL_SMAX stages bound what a plan can hold.
-constexpr int RKL_SMAX = 64;
-struct RklPlan {
-    int s, overflow;
-    int active[RKL_SMAX + 1];   // active[j] = 1 for stages 1 <= j <= s
-    double coef[4 * RKL_SMAX];  // mu, nu, mu~, gamma~ of stage j at 4 (j - 1)
-};
-
 // The recurrence of rkl2_coeffs (host), operation for operation and without
 // contraction into FMAs, so that both give the same doubles.
-__global__ void k_rkl2_plan(const TimeState *__restrict__ ts, int smax, RklPlan *__restrict__ plan) {
+__global__ void k_rkl2_plan(const TimeState *__restrict__ ts, int smax, RklPlan *__restrict__ plan, Batch bt) {
 #pragma clang fp contract(off)
     if (threadIdx.x != 0) return;
+    ts = rebase(bt, 0, ts);
+    plan = rebase(bt, 1, plan);
     const double scrh = ts->dt / ts->dt_sts;
     const double s_str = 4.0 * (1.0 + 2.0 * scrh) / (1.0 + sqrt(9.0 + 16.0 * scrh));
     // 1 + (int)s_str as on the host; past smax (or NaN) the plan is cut at smax
@@ -238,7 +226,19 @@ __global__ __launch_bounds__(256) void k_rkl2(const cpx<T> *__restrict__ Q, cpx<
                                               cpx<double> *__restrict__ Uc0, cpx<double> *__restrict__ Uc1,
                                               cpx<double> *__restrict__ dU0, RklConst c, int stage, double mu,
                                               double nu, double dtmut, double gdt, const RklPlan *__restrict__ plan,
-                                              const TimeState *__restrict__ ts) {
+                                              const TimeState *__restrict__ ts, Batch bt) {
+    Q = rebase(bt, 0, Q);
+    U = rebase(bt, 1, U);
+    Uc0 = rebase(bt, 2, Uc0);
+    Uc1 = rebase(bt, 3, Uc1);
+    dU0 = rebase(bt, 4, dU0);
+    plan = rebase(bt, 5, plan);
+    ts = rebase(bt, 6, ts);
+    if (const BoxRow *br = batch_row(bt)) {
+        c.nu = br->nu;
+        c.nu_m = br->nu_m;
+        c.nu_th = br->nu_th;
+    }
     if (plan != nullptr) {
         if (!plan->active[stage]) return;
         const double *cf = plan->coef + 4 * (stage - 1);
@@ -643,6 +643,10 @@ __global__ __launch_bounds__(256) void k_unshear(const cpx<double> *__restrict__
 
 namespace spk {
 bool fused_supported_nx(int nx);
+bool rkl_fused_supported_nx(int nx);
+int launch_fwdx_rkl2(spk_ctx ctx, int bulk, int nx, const void *Q, void *U, void *Uc0, void *Uc1, void *dU0, void *W,
+                     const RklConst &c, int stage, double mu, double nu, double dtmut, double gdt, int inv,
+                     double scale, const void *tw, const RklPlan *plan, const TimeState *ts);
 int launch_fwdx_epilogue(spk_ctx ctx, int bulk, int nx, int ly, int mode, int ef, const void *P, void *U, void *S,
                          void *W, const FusedConst &fc, const void *tw, const TimeState *ts, int ngroups);
 }  // namespace spk
@@ -723,13 +727,15 @@ int launch_inv_x(spk_solver s, const InvProg &prog, int nslots, void *W, double 
     const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
     const cpx<T> *tw = twp<T>(s->twb, s->twb.off_x);
     KGeom kg = kgeom(s, kxshift);
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->U, W, s->ctx->gate}, &bt));
 #define SPK_INV_CASE(NN)                                                                          \
     case NN: {                                                                                    \
         constexpr int TC = strided_tcols(NN);                                                     \
-        dim3 grid((unsigned)(((long long)g.ny * g.nzh + TC - 1) / TC), nslots);                  \
+        dim3 grid((unsigned)(((long long)g.ny * g.nzh + TC - 1) / TC), nslots, nboxes(s->ctx));  \
         k_inv_x<T, NN><<<grid, TC * strided_tpl(NN), 0, s->ctx->stream>>>(s->U, (cpx<T> *)W,     \
                                                                            g.ntc, prog, kg, scale, tw, \
-                                                                           s->ctx->gate);              \
+                                                                           s->ctx->gate, bt);          \
         break;                                                                                    \
     }
     switch (g.nx) {
@@ -756,19 +762,22 @@ int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax,
     const cpx<T> *wr = (const cpx<T> *)s->Breal;
     cpx<T> *pp = (cpx<T> *)P;
     hipStream_t st = s->ctx->stream;
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {W, s->Breal, P, pmax, s->ctx->gate}, &bt));
+    const dim3 grid(nb, 1, nboxes(s->ctx));
     switch (prog) {
-        case ZP_U: k_zmid<T, NZ, ZP_U><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
-        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
-        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
+        case ZP_U: k_zmid<T, NZ, ZP_U><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
+        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
+        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
         case ZP_U | ZP_B | ZP_TH:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
             break;
         case ZP_U | ZP_B | ZP_TH | ZP_G:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
+            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
             break;
-        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate); break;
+        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
         case ZP_B | ZP_G | ZP_BR:
-            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<nb, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate);
+            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
             break;
         default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
     }
@@ -794,27 +803,31 @@ int launch_zmid(spk_solver s, int prog, const void *W, void *P, double *pmax, un
 }
 
 template <class T, int MODE>
-void launch_epilogue_mode(spk_solver s, const EpiConst &c) {
+int launch_epilogue_mode(spk_solver s, const EpiConst &c) {
     const unsigned nb = nblk(s->g.ntc);
     hipStream_t st = s->ctx->stream;
     const cpx<T> *P = (const cpx<T> *)s->P;
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {P, s->U, s->S, s->dU, s->ts}, &bt));
+    const dim3 grid(nb, 1, nboxes(s->ctx));
     switch (s->ly) {
-        case LY_HEAT: k_epilogue<T, LY_HEAT, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
-        case LY_HD: k_epilogue<T, LY_HD, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
-        case LY_HD_TH: k_epilogue<T, LY_HD_TH, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
-        case LY_MHD: k_epilogue<T, LY_MHD, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
-        default: k_epilogue<T, LY_MHD_TH, MODE><<<nb, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts); break;
+        case LY_HEAT: k_epilogue<T, LY_HEAT, MODE><<<grid, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts, bt); break;
+        case LY_HD: k_epilogue<T, LY_HD, MODE><<<grid, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts, bt); break;
+        case LY_HD_TH: k_epilogue<T, LY_HD_TH, MODE><<<grid, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts, bt); break;
+        case LY_MHD: k_epilogue<T, LY_MHD, MODE><<<grid, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts, bt); break;
+        default: k_epilogue<T, LY_MHD_TH, MODE><<<grid, 256, 0, st>>>(P, s->U, s->S, s->dU, c, s->ts, bt); break;
     }
+    return SPK_OK;
 }
 
 // mode 0: store dU; 1, 2, 3: RK3 stage update
 template <class T>
 int launch_epilogue(spk_solver s, const EpiConst &c, int mode) {
     switch (mode) {
-        case 0: launch_epilogue_mode<T, 0>(s, c); break;
-        case 1: launch_epilogue_mode<T, 1>(s, c); break;
-        case 2: launch_epilogue_mode<T, 2>(s, c); break;
-        default: launch_epilogue_mode<T, 3>(s, c); break;
+        case 0: SPK_TRY((launch_epilogue_mode<T, 0>(s, c))); break;
+        case 1: SPK_TRY((launch_epilogue_mode<T, 1>(s, c))); break;
+        case 2: SPK_TRY((launch_epilogue_mode<T, 2>(s, c))); break;
+        default: SPK_TRY((launch_epilogue_mode<T, 3>(s, c))); break;
     }
     SPK_LAUNCH_CHECK();
     return SPK_OK;
@@ -944,13 +957,18 @@ int dt_kernel(spk_solver s) {
     c.cfl = p.cfl; c.cfl_par = p.cfl_par; c.cfl_sts = p.cfl_sts; c.safety_source = p.safety_source;
     c.t_final = p.t_final; c.toutput_flow = p.toutput_flow;
     c.nblocks = s->zmid_blocks;
-    k_dt<<<1, 256, 0, s->ctx->stream>>>(s->pmax, c, s->ts);
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->pmax, s->ts}, &bt));
+    k_dt<<<dim3(1, 1, nboxes(s->ctx)), 256, 0, s->ctx->stream>>>(s->pmax, c, s->ts, bt);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
 
 int clock(spk_solver s, int add_dt, int add_step, int snap, int set_tremap, double tremap) {
-    k_clock<<<1, 1, 0, s->ctx->stream>>>(s->ts, add_dt, add_step, snap, s->p.toutput_flow, set_tremap, tremap);
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->ts}, &bt));
+    k_clock<<<dim3(1, 1, nboxes(s->ctx)), 1, 0, s->ctx->stream>>>(s->ts, add_dt, add_step, snap, s->p.toutput_flow,
+                                                                   set_tremap, tremap, bt);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
@@ -965,7 +983,10 @@ int clean_divergence(spk_solver s) {
     const int f0 = (s->p.incompressible && !s->p.heat_equation) ? s->VEL : -1;
     const int f1 = s->p.mhd && f0 >= 0 ? s->MAG : -1;
     if (f0 < 0 && f1 < 0) return SPK_OK;
-    k_clean_div<<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>(s->U, s->g.ntc, kgeom(s, s->tremap * s->p.shear), f0, f1);
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->U}, &bt));
+    k_clean_div<<<dim3(nblk(s->g.ntc), 1, nboxes(s->ctx)), 256, 0, s->ctx->stream>>>(
+        s->U, s->g.ntc, kgeom(s, s->tremap * s->p.shear), f0, f1, bt);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
@@ -1066,8 +1087,10 @@ template <class T>
 int rkl2_stage_launch(spk_solver s, int stage, double mu, double nu, double dtmut, double gdt) {
     RklConst c = s->rkl;
     c.kg = kgeom(s, s->tremap * s->p.shear);
-    k_rkl2<T><<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>((const cpx<T> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c,
-                                                           stage, mu, nu, dtmut, gdt, nullptr, nullptr);
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->P2, s->U, s->rUc0, s->rUc1, s->rdU0}, &bt));
+    k_rkl2<T><<<dim3(nblk(s->g.ntc), 1, nboxes(s->ctx)), 256, 0, s->ctx->stream>>>(
+        (const cpx<T> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c, stage, mu, nu, dtmut, gdt, nullptr, nullptr, bt);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }
@@ -1120,6 +1143,46 @@ int rkl2_front_cached(spk_solver s) {
     return SPK_OK;
 }
 
+// The fused RKL2 stage (spk_rklfused.hip): anisotropic conduction of theta as
+// the only sts variable, B's real-space rows cached for the cycle, on a grid the
+// fused x kernel is built for, and the fused back end not switched off.
+bool rkl_fused(spk_solver s) {
+    return s->rkl_aniso && s->rkl_cache_b && s->nsts == 1 && !s->p.shearing && rkl_fused_supported_nx(s->p.nx) &&
+           s->fused_mode != 0;
+}
+
+// the part of a fused RKL2 stage before its forward x pass: (stage 1 only) the
+// inverse x pass of i k theta, then the inverse y pass, the flux in the z
+// kernel and the forward y pass of the flux
+int rkl2_front_fused(spk_solver s, bool with_inv_x) {
+    const Grid3 &g = s->g;
+    const size_t fb = (size_t)g.ntc * esize(s->bulk);
+    void *wg = (char *)s->W2 + 3 * fb;  // the i k theta slots of the fused z program
+    if (with_inv_x) {
+        if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_g, 3, wg, 0.0));
+        else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_g, 3, wg, 0.0));
+    }
+    PassArgs a{wg, wg, 3, g.ntc, g.ntc, 1.0};
+    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
+    else SPK_TRY(launch_zmid<float>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
+    PassArgs b{s->P2, s->P2, 3, g.ntc, g.ntc, 1.0};
+    return launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b);
+}
+
+// forward x pass of the flux + stage update + (inv) the next stage's inverse x pass
+int rkl2_back_fused(spk_solver s, int stage, double mu, double nu, double dtmut, double gdt, int inv,
+                    const RklPlan *plan) {
+    const Grid3 &g = s->g;
+    const size_t fb = (size_t)g.ntc * esize(s->bulk);
+    RklConst c = s->rkl;
+    c.kg = kgeom(s, 0.0);
+    const void *tw = (const char *)s->twb.d + (size_t)s->twb.off_x * esize(s->bulk);
+    return launch_fwdx_rkl2(s->ctx, s->bulk, g.nx, s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, (char *)s->W2 + 3 * fb, c,
+                            stage, mu, nu, dtmut, gdt, inv, 1.0 / ((double)g.nx * g.ny * g.nz), tw, plan,
+                            plan ? s->ts : nullptr);
+}
+
 int rkl2_cycle(spk_solver s) {
     if (s->nsts == 0) return SPK_OK;
     TimeState h;
@@ -1127,6 +1190,14 @@ int rkl2_cycle(spk_solver s) {
     std::vector<double> coef;
     const int sr = rkl2_coeffs(h.dt, h.dt_sts, coef);
     if (s->rkl_aniso && s->rkl_cache_b) SPK_TRY(rkl2_b_real(s));
+    if (rkl_fused(s)) {
+        for (int stage = 1; stage <= sr; stage++) {
+            SPK_TRY(rkl2_front_fused(s, stage == 1));
+            const double *c = &coef[4 * (size_t)(stage - 1)];
+            SPK_TRY(rkl2_back_fused(s, stage, c[0], c[1], c[2] * h.dt, c[3] * h.dt, stage < sr, nullptr));
+        }
+        return SPK_OK;
+    }
     for (int stage = 1; stage <= sr; stage++) {
         if (s->rkl_aniso) SPK_TRY(s->rkl_cache_b ? rkl2_front_cached(s) : rkl2_front(s));
         const double *c = &coef[4 * (size_t)(stage - 1)];
@@ -1144,9 +1215,24 @@ int rkl2_cycle(spk_solver s) {
 int rkl2_cycle_device(spk_solver s) {
     if (s->nsts == 0) return SPK_OK;
     hipStream_t st = s->ctx->stream;
-    k_rkl2_plan<<<1, 64, 0, st>>>(s->ts, s->rkl_smax, s->rkl_plan);
+    Batch bp;
+    SPK_TRY(make_batch(s->ctx, {s->ts, s->rkl_plan}, &bp));
+    k_rkl2_plan<<<dim3(1, 1, nboxes(s->ctx)), 64, 0, st>>>(s->ts, s->rkl_smax, s->rkl_plan, bp);
     SPK_LAUNCH_CHECK();
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, s->rkl_plan, s->ts}, &bt));
+    const dim3 grid(nblk(s->g.ntc), 1, nboxes(s->ctx));
     if (s->rkl_aniso && s->rkl_cache_b) SPK_TRY(rkl2_b_real(s));
+    if (rkl_fused(s)) {
+        for (int stage = 1; stage <= s->rkl_smax; stage++) {
+            s->ctx->gate = &s->rkl_plan->active[stage];  // a device address; never read on the host
+            const int rc = rkl2_front_fused(s, stage == 1);
+            s->ctx->gate = nullptr;
+            SPK_TRY(rc);
+            SPK_TRY(rkl2_back_fused(s, stage, 0.0, 0.0, 0.0, 0.0, stage < s->rkl_smax, s->rkl_plan));
+        }
+        return SPK_OK;
+    }
     RklConst c = s->rkl;
     c.kg = kgeom(s, s->tremap * s->p.shear);
     for (int stage = 1; stage <= s->rkl_smax; stage++) {
@@ -1157,11 +1243,11 @@ int rkl2_cycle_device(spk_solver s) {
             SPK_TRY(rc);
         }
         if (s->bulk == SPK_F64)
-            k_rkl2<double><<<nblk(s->g.ntc), 256, 0, st>>>((const cpx<double> *)s->P2, s->U, s->rUc0, s->rUc1,
-                                                           s->rdU0, c, stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts);
+            k_rkl2<double><<<grid, 256, 0, st>>>((const cpx<double> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c,
+                                                 stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts, bt);
         else
-            k_rkl2<float><<<nblk(s->g.ntc), 256, 0, st>>>((const cpx<float> *)s->P2, s->U, s->rUc0, s->rUc1,
-                                                          s->rdU0, c, stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts);
+            k_rkl2<float><<<grid, 256, 0, st>>>((const cpx<float> *)s->P2, s->U, s->rUc0, s->rUc1, s->rdU0, c,
+                                                stage, 0.0, 0.0, 0.0, 0.0, s->rkl_plan, s->ts, bt);
         SPK_LAUNCH_CHECK();
     }
     return SPK_OK;
@@ -1601,6 +1687,7 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
             SPK_TRY(hydro_mhd_advance(s));
         }
         if (s->force_n > 0) {
+            SPK_REQUIRE(!s->ctx->rows, SPK_ERR_STATE, "spk_main_loop: forcing is not batched");
             k_force<<<(s->force_n + 255) / 256, 256, 0, s->ctx->stream>>>(s->U, s->g.ntc, s->VX, s->force_idx,
                                                                           s->force_val, s->force_n, s->ts);
             SPK_LAUNCH_CHECK();
@@ -2171,6 +2258,64 @@ void solver_advance_host(spk_solver s, int n) {
     s->step += n;
     s->products_valid = false;
 }
+
+void solver_box_row(spk_solver s, BoxRow *row, size_t *sizes) {
+    memset(row, 0, sizeof(*row));
+    for (int k = 0; k < BOX_SLOTS; k++) sizes[k] = 0;
+    const size_t f64 = (size_t)s->g.ntc * 16, fb = (size_t)s->g.ntc * esize(s->bulk);
+    auto put = [&](int k, const void *p, size_t n) {
+        row->p[k] = static_cast<char *>(const_cast<void *>(p));
+        sizes[k] = p ? n : 0;
+    };
+    put(SL_U, s->U, f64 * s->nf);
+    put(SL_S, s->S, f64 * s->nf);
+    put(SL_DU, s->dU, f64 * s->nf);
+    put(SL_REAL, s->realbuf, f64 * s->nf);
+    put(SL_W, s->W, fb * (s->nin > 0 ? s->nin : 1));
+    put(SL_P, s->P, fb * (s->nout > 0 ? s->nout : 1));
+    put(SL_W2, s->W2, fb * 6);
+    put(SL_P2, s->P2, fb * 3);
+    put(SL_BREAL, s->Breal, fb * 3);
+    put(SL_TS, s->ts, sizeof(TimeState));
+    put(SL_PMAX, s->pmax, sizeof(double) * 6 * (size_t)((long long)s->p.nx * s->p.ny + 1));
+    put(SL_UC0, s->rUc0, f64 * s->nsts);
+    put(SL_UC1, s->rUc1, f64 * s->nsts);
+    put(SL_DU0, s->rdU0, f64 * s->nsts);
+    put(SL_PLAN, s->rkl_plan, sizeof(RklPlan));
+    row->nu = s->p.nu;
+    row->nu_m = s->p.nu_m;
+    row->nu_th = s->p.nu_th;
+    row->reynolds_ani = s->p.reynolds_ani;
+}
+
+bool solver_same_shape(spk_solver a, spk_solver b) {
+    spk_params pa = a->p, pb = b->p;
+    pa.nu = pb.nu = 0.0;
+    pa.nu_m = pb.nu_m = 0.0;
+    pa.nu_th = pb.nu_th = 0.0;
+    pa.reynolds_ani = pb.reynolds_ani = 0.0;
+    const bool ints = pa.nx == pb.nx && pa.ny == pb.ny && pa.nz == pb.nz && pa.incompressible == pb.incompressible &&
+                      pa.heat_equation == pb.heat_equation && pa.boussinesq == pb.boussinesq &&
+                      pa.stratification == pb.stratification && pa.strat_direction == pb.strat_direction &&
+                      pa.mhd == pb.mhd && pa.anisotropic_diffusion == pb.anisotropic_diffusion &&
+                      pa.supertimestepping == pb.supertimestepping && pa.shearing == pb.shearing &&
+                      pa.rotating == pb.rotating && pa.with2d == pb.with2d && pa.antialiasing == pb.antialiasing &&
+                      pa.sts_mask == pb.sts_mask && pa.safety_sts == pb.safety_sts;
+    const bool dbls = pa.lx == pb.lx && pa.ly == pb.ly && pa.lz == pb.lz && pa.N2 == pb.N2 &&
+                      pa.OmegaT2 == pb.OmegaT2 && pa.omega == pb.omega && pa.shear == pb.shear && pa.cfl == pb.cfl &&
+                      pa.cfl_par == pb.cfl_par && pa.cfl_sts == pb.cfl_sts && pa.safety_source == pb.safety_source &&
+                      pa.t_initial == pb.t_initial && pa.t_final == pb.t_final && pa.toutput_flow == pb.toutput_flow;
+    return ints && dbls && a->bulk == b->bulk && a->rkl_device == b->rkl_device && a->force_n == 0 &&
+           b->force_n == 0 && !a->p.shearing && (a->W2 != nullptr) == (b->W2 != nullptr) &&
+           (a->Breal != nullptr) == (b->Breal != nullptr);
+}
+
+int solver_follow_back_end(spk_solver s, spk_solver lead) {
+    s->fused_mode = lead->fused_mode;
+    s->fused_faster = lead->fused_faster;
+    s->fused_ef = lead->fused_ef;
+    return SPK_OK;
+}
 }  // namespace spk
 
 extern "C" int spk_solver_set_rkl_device(spk_solver s, int on) {
@@ -2345,8 +2490,10 @@ extern "C" int spk_clean_divergence_ptr(spk_solver s, void *complex_fields) {
 extern "C" int spk_project(spk_solver s, void *complex_vec) {
     SPK_CHECK_SOLVER(s);
     SPK_CHECK_PTR(complex_vec, "complex_vec");
+    Batch bt;
+    SPK_TRY(make_batch(s->ctx, {complex_vec}, &bt));
     k_clean_div<<<nblk(s->g.ntc), 256, 0, s->ctx->stream>>>(static_cast<cpx<double> *>(complex_vec), s->g.ntc,
-                                                            kgeom(s, s->tremap * s->p.shear), 0, -1);
+                                                            kgeom(s, s->tremap * s->p.shear), 0, -1, bt);
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }

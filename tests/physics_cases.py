@@ -43,6 +43,12 @@ CASES = {
                                        reynolds_ani=20.0, nu=0.01, nu_m=0.01, nu_th=0.01, cfl=0.5,
                                        supertimestepping=1, sts_mask=1 << 6, safety_sts=20, cfl_par=1.0,
                                        lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T), fields="ubt", b0=1.0),
+    "mhd_bouss_rkl2_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1,
+                                           stratification=1, strat_direction=2, anisotropic_diffusion=1, N2=0.1,
+                                           OmegaT2=1.0, reynolds_ani=30.0, nu=0.01, nu_m=0.01, nu_th=0.01,
+                                           cfl=0.5, supertimestepping=1, sts_mask=1 << 6, safety_sts=20,
+                                           cfl_par=1.0, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T),
+                               fields="ubt", b0=1.0),
     "shear_mhd": dict(params=dict(nx=16, ny=16, nz=8, incompressible=1, mhd=1, shearing=1, rotating=1,
                                   shear=1.5, omega=1.0, nu=1e-3, nu_m=1e-3, cfl=0.5, lx=1.0, ly=1.5,
                                   lz=1.0, **BIG_T), fields="ub"),

@@ -3,10 +3,13 @@
 - A device-planned RKL2 cycle (the recurrence of rkl.cu:466-585 evaluated by a
   one-thread kernel, extra stages returning at once) gives the host-planned
   cycle's state, and its stage count is the recurrence's.
-- Every box of an ensemble (uneven grids and physics, several lanes, two steps
-  per graph replay) ends where the same solver run on its own ends, with the
-  graph and without it, and where the FP64 oracle ends.
-- The lane autotune leaves every box's state and clock as it found them.
+- Every box of an ensemble (uneven grids and physics, two steps per graph
+  replay) ends where the same solver run on its own ends, with the graph and
+  without it, and where the FP64 oracle ends.
+- Batched launches (one launch per kernel for every box of a shape group, the
+  box in blockIdx.z) give every box the bits that box-by-box launches give it,
+  for any cut of the groups into batches.
+- The layout autotune leaves every box's state and clock as it found them.
 """
 import numpy as np
 import pytest
@@ -139,6 +142,42 @@ def test_ensemble_fp32_and_recapture(ctx):
         assert relerr(got[i], sim.fields) < 2e-4, (i, relerr(got[i], sim.fields))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batched_launches_match_box_by_box(ctx, prec):
+    """Shape groups of boxes() (two 16^3 MHD boxes, one 32x16x16, two 32x32x2
+    HBI boxes with RKL2): one batch per group, two batches per group, and box by
+    box give bit-identical states and clocks after 6 steps (graph on, then 2
+    more steps without it)."""
+    bx = boxes()
+    states = [box_state(p, 400 + i) for i, p in enumerate(bx)]
+    results = {}
+    # one ensemble throughout: every box keeps the back end its group's first box
+    # was tuned to, so the three launch layouts run the same arithmetic
+    ens = spk.Ensemble(bx, prec, lanes=2)
+    assert ens.layout() == dict(groups=3, split=1, batches=3, lanes=3)
+    for split in (1, 2, 0):
+        print("# split %d, %s" % (split, prec), flush=True)
+        ens.set_split(split)
+        lay = ens.layout()
+        assert lay["split"] == split and lay["batches"] == {1: 3, 2: 5, 0: 0}[split]
+        ens.set_graph(True)
+        for i, st in enumerate(states):
+            ens.set_fields(i, st)
+            ens.boxes[i].set_time(0.0)
+        ens.run(6)
+        ens.set_graph(False)
+        ens.run(2)
+        results[split] = ([ens.get_fields(i) for i in range(len(bx))], [ens.time(i) for i in range(len(bx))])
+    ens.destroy()
+    for split in (1, 2):
+        for i in range(len(bx)):
+            assert np.array_equal(results[split][0][i], results[0][0][i]), (split, i)
+            assert results[split][1][i] == results[0][1][i], (split, i)
+    # the HBI boxes took RKL2 cycles of their own stage counts
+    t3, t4 = results[1][1][3], results[1][1][4]
+    assert t3["dt_sts"] != t4["dt_sts"]
+
+
 def test_ensemble_tune_restores_state(ctx):
     bx = boxes()
     states = [box_state(p, 300 + i) for i, p in enumerate(bx)]
@@ -148,8 +187,8 @@ def test_ensemble_tune_restores_state(ctx):
     ens.run(2)
     before = [ens.get_fields(i) for i in range(len(bx))]
     tb = [ens.time(i) for i in range(len(bx))]
-    lanes, ms = ens.tune(probe=2)
-    assert lanes in (1, 2, 4) and ms > 0 and ens.lanes() == lanes
+    split, ms = ens.tune(probe=2)
+    assert split in (1, 2, 4) and ms > 0 and ens.layout()["split"] == split
     for i in range(len(bx)):
         assert np.array_equal(ens.get_fields(i), before[i])
         t = ens.time(i)

@@ -122,6 +122,51 @@ def test_rkl2_cycle(ctx):
     assert s > 2  # the cycle really is multi-stage here
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("device_plan", [False, True])
+def test_fused_rkl2_stage_matches_unfused_and_oracle(ctx, prec, device_plan):
+    """Anisotropic conduction of theta by RKL2 on a grid the fused x kernel takes
+    (NX = 32): one cycle, and four main-loop steps (cycles before and after RK3),
+    with the fused stage (forward x pass of the flux + update + next inverse x
+    pass in one kernel) against the unfused stages and the oracle; host- and
+    device-planned cycles."""
+    name = "mhd_bouss_rkl2_x32"
+    p = CASES[name]["params"]
+    u0 = initial_state(name, 3)
+    fus = spk.Solver(ctx, prec, **p)
+    unf = spk.Solver(ctx, prec, **p)
+    unf.set_fused(0)
+    for s in (fus, unf):
+        s.set_rkl_device(device_plan)
+        s.set_fields(u0)
+    sim = orc.Sim(**p)
+    sim.set_fields(u0)
+    sim.compute_dt()
+    for s in (fus, unf):
+        s.compute_dt()
+        s.rkl2_cycle()
+    sim.rkl2_cycle()
+    n, _ = orc.rkl2_coeffs(sim.time()["dt"], sim.time()["dt_sts"])
+    assert n > 2
+    tol_pair = 1e-13 if prec == "f64" else 2e-5
+    tol_orc = 1e-12 if prec == "f64" else 2e-4
+    a, b = fus.get_fields(), unf.get_fields()
+    assert relerr(a, b) < tol_pair, relerr(a, b)
+    assert relerr(a, sim.fields) < tol_orc, relerr(a, sim.fields)
+    for s in (fus, unf):
+        s.set_fields(u0)
+        s.set_time(0.0)
+        s.main_loop(4)
+    sim = orc.Sim(**p)
+    sim.set_fields(u0)
+    sim.main_loop(4)
+    a, b = fus.get_fields(), unf.get_fields()
+    assert relerr(a, b) < (1e-12 if prec == "f64" else 2e-5), relerr(a, b)
+    assert relerr(a, sim.fields) < TOL_STEPS[prec], relerr(a, sim.fields)
+    fus.destroy()
+    unf.destroy()
+
+
 def test_taylor_green_64_100_steps(ctx):
     """Config 1: HD Taylor-Green, 64^3, 2pi box, nu=0.01, CFL 0.5, 2/3 dealiasing,
     100 steps; kinetic energy and dissipation 2 nu <w^2> against the oracle."""
