@@ -228,7 +228,7 @@ constexpr size_t DW_LDS = (size_t)(CH * DZS + CH * XPL) * sizeof(bf16);
 
 template <int CIN>
 __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
-                                                 float *__restrict__ part, int n) {
+                                                 float *__restrict__ part, float *__restrict__ bpart, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     bf16 *Ds = reinterpret_cast<bf16 *>(smem);  // [32 co][DZS]: point (bx, by, z) at (bx 4 + by) 32 + z
     bf16 *Xs = Ds + CH * DZS;                   // [32 ci][XPL]: (hx, hy, z0 - 8 + e) at (hx 6 + hy) 48 + e
@@ -240,6 +240,11 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
     for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[k][r] = 0.0f;
+    // bias gradient: the block's sum of dZ per output channel, read from the
+    // bricks already in LDS; thread tid sums channel tid / 16, 8-point chunks
+    // (tid % 16) + 16 r of each brick (consecutive lanes, consecutive 16 bytes)
+    const int bco = tid >> 4, bseg = tid & 15;
+    float bacc = 0.0f;
     const int nbz = n / 32, nby = n / 4;
     const int nb = (n / 4) * nby * nbz;
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
@@ -265,6 +270,12 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
             for (int j = 0; j < 8; j++) Xs[(8 * q + j) * XPL + e] = v[j];
         }
         __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Ds + bco * DZS + (bseg + 16 * r) * 8);
+#pragma unroll
+            for (int j = 0; j < 8; j++) bacc += (float)v[j];
+        }
         for (int s = 0; s < 32; s++) {
             const int run = 2 * s + half, col = run >> 2, zr = (run & 3) * 8;
             const int bxl = col >> 2, byl = col & 3;
@@ -297,6 +308,9 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
             }
         }
     }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) bacc += __shfl_xor(bacc, o, 64);
+    if (bseg == 0) bpart[(size_t)blockIdx.x * CH + bco] = bacc;
     // D[co][ci]: lane holds ci = li, co = 8 (r / 4) + 4 half + r % 4
     float *pb = part + (size_t)blockIdx.x * NTAP * CH * CH;
 #pragma unroll
@@ -310,32 +324,6 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
             }
         }
     }
-}
-
-// bias gradient partials: part[block][32] = sum over the block's points of dZ.
-// Thread tid reads 8-channel chunk tid % 4 of its points (16-byte coalesced
-// loads); the wave sums lanes of equal chunk with shuffles, then the 4 waves.
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *__restrict__ dZ, size_t P, float *__restrict__ part) {
-    __shared__ float sm[4][CH];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    float acc[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = 0.0f;
-    const size_t nchunk = P * 4;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + tid; i < nchunk; i += (size_t)gridDim.x * blockDim.x) {
-        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + i * 8);
-#pragma unroll
-        for (int j = 0; j < 8; j++) acc[j] += (float)v[j];
-    }
-#pragma unroll
-    for (int o = 4; o < 64; o <<= 1)
-#pragma unroll
-        for (int j = 0; j < 8; j++) acc[j] += __shfl_xor(acc[j], o, 64);
-    if (lane < 4)
-#pragma unroll
-        for (int j = 0; j < 8; j++) sm[w][8 * lane + j] = acc[j];
-    __syncthreads();
-    if (tid < CH) part[(size_t)blockIdx.x * CH + tid] = sm[0][tid] + sm[1][tid] + sm[2][tid] + sm[3][tid];
 }
 
 __global__ void k_fill0(float *__restrict__ p, int count) {
@@ -680,8 +668,8 @@ int forward(spk_closure c, const void *u, bool train) {
 int layer_grads(spk_closure c, int l, const bf16 *X, const bf16 *dz) {
     hipStream_t st = c->ctx->stream;
     const unsigned g = conv_grid(c);
-    if (CIN_L[l] == 16) k_conv_dw<16><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->n);
-    else k_conv_dw<32><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->n);
+    if (CIN_L[l] == 16) k_conv_dw<16><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->bpart, c->n);
+    else k_conv_dw<32><<<g, CONV_T, DW_LDS, st>>>(X, dz, c->part, c->bpart, c->n);
     const int cnt = NTAP * CH * CH;
     // the [27][32][32] sums land in grd for cin = 32; for cin = 16 take ci < 16
     float *gw = c->grd + param_offset(l);
@@ -691,9 +679,9 @@ int layer_grads(spk_closure c, int l, const bf16 *X, const bf16 *dz) {
     k_sum_parts<<<sg, 256, 0, st>>>(c->part, (int)g, cnt, sum_to);
     if (CIN_L[l] != 32)
         SPK_HIP(hipMemcpy2DAsync(gw, 16 * 4, c->out, 32 * 4, 16 * 4, NTAP * CH, hipMemcpyDeviceToDevice, st));
-    k_bias_grad<<<NBLK, 256, 0, st>>>(dz, c->P, c->bpart);
+    // the bias partials came from the same kernel (k_conv_dw), one per block
     k_fill0<<<1, CH, 0, st>>>(gw + w_count(l), CH);
-    k_sum_parts<<<dim3(1, SUM_SLICES), CH, 0, st>>>(c->bpart, NBLK, CH, gw + w_count(l));
+    k_sum_parts<<<dim3(1, SUM_SLICES), CH, 0, st>>>(c->bpart, (int)g, CH, gw + w_count(l));
     SPK_LAUNCH_CHECK();
     return SPK_OK;
 }

@@ -309,13 +309,13 @@ int main(int argc, char **argv) {
     mkdir(R.data_dir.c_str(), 0755);
 
     CALL(spk_ctx_create(0, nullptr, &R.ctx));
-    CALL(spk_solver_create(R.ctx, &p, SPK_F64, &R.sol));
+    CALL(spk_solver_create(R.ctx, &p, R.rp.bulk, &R.sol));
     CALL(spk_solver_info(R.sol, R.vars, &R.ntc));
     R.nf = R.vars[9];
     R.names = field_names(p);
     CALL(spk_solver_buffers(R.sol, &R.d_fields, &R.d_dfields, &R.d_real));
-    std::printf("grid %d x %d x %d, box %g x %g x %g, %d fields, output in %s\n", p.nx, p.ny, p.nz, p.lx, p.ly, p.lz,
-                R.nf, R.rp.output_dir.c_str());
+    std::printf("grid %d x %d x %d, box %g x %g x %g, %d fields, %s bulk, output in %s\n", p.nx, p.ny, p.nz, p.lx,
+                p.ly, p.lz, R.nf, R.rp.bulk == SPK_F32 ? "FP32" : "FP64", R.rp.output_dir.c_str());
 
     const bool restart = cli.restart_num != -2 || R.rp.restart;
     if (restart) R.read_snapshot(cli.restart_num == -2 ? -1 : cli.restart_num);

@@ -17,6 +17,7 @@ const char *cli_usage() {
            "  -t HH MM SS   maximum wall-clock time (95% of it is used)\n"
            "  -n NX NY NZ   override the grid size\n"
            "  --with-2d     force a 2D (x, y) problem\n"
+           "  --fp32-bulk   transforms and real-space products in FP32 (state, RK3 and RKL2 stay FP64)\n"
            "  --print-params  print the resolved parameters as JSON and exit\n";
 }
 
@@ -76,6 +77,8 @@ bool parse_cli(int argc, char **argv, CliArgs *a, std::string *err) {
             a->with2d = true;
         } else if (o == "--print-params") {
             a->print_params = true;
+        } else if (o == "--fp32-bulk") {
+            a->fp32_bulk = true;
         } else {
             *err = "unknown option " + o;
             return false;
@@ -156,6 +159,7 @@ bool read_run_params(const Config &c, const CliArgs &cli, RunParams *r, std::str
         r->max_walltime_elapsed = 42;
         if (cli.max_hours > 0.0) r->max_walltime_elapsed = cli.max_hours;
     }
+    r->bulk = cli.fp32_bulk ? SPK_F32 : SPK_F64;
     if (!c.lookup_int("code.interface_check", &r->interface_check)) r->interface_check = 5;
     if (!c.lookup_bool("code.interface_output_file", &r->interface_output_file)) r->interface_output_file = 0;
     if (!c.lookup_bool("code.force_symmetries", &r->force_symmetries)) r->force_symmetries = 0;
@@ -264,7 +268,8 @@ std::string params_json(const RunParams &r) {
            jstr(r.output_dir) + ", \"sts_algorithm\": " + jstr(r.sts_algorithm) + ", \"sts_variables\": " +
            jlist(r.sts_variables) + ", \"timevar_vars\": " + jlist(r.timevar_vars) + ", \"spectra_vars\": " +
            jlist(r.spectra_vars) + ", \"user_timevar_vars\": " + jlist(r.user_timevar_vars) +
-           ", \"fields\": " + jlist(field_names(p)) + "}";
+           ", \"fields\": " + jlist(field_names(p)) + ", \"bulk\": " + jstr(r.bulk == SPK_F32 ? "f32" : "f64") +
+           "}";
 }
 
 int check_params(RunParams *r, std::vector<std::string> *msgs) {

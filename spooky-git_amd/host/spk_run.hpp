@@ -24,6 +24,7 @@ struct CliArgs {
     bool with2d = false;
     bool help = false;
     bool print_params = false;  // print the resolved parameters as JSON and exit (no GPU used)
+    bool fp32_bulk = false;     // transforms and products in FP32, state and RK3/RKL2 in FP64
 };
 // Parses argv; returns false with a message on a bad command line.
 bool parse_cli(int argc, char **argv, CliArgs *a, std::string *err);
@@ -48,6 +49,7 @@ struct RunParams {
     double vortex_a = 0, vortex_b = 0, per_amplitude_large = 0, noise_cut_length = 0;
     double per_amplitude_large_2D = 0, noise_cut_length_2D = 0, per_amplitude_noise = 0;
     double bx0 = 0, by0 = 0, bz0 = 0;
+    int bulk = SPK_F64;           // precision of the transforms and products (--fp32-bulk)
 };
 
 // Fills r from cfg and the command line overrides; returns false with a

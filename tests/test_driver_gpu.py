@@ -68,6 +68,20 @@ def test_taylor_green_vortex(tmp_path):
     assert np.max(np.abs(rows[:, 6])) < 1e-12
 
 
+def test_taylor_green_vortex_fp32_bulk(tmp_path):
+    """The same case with `--fp32-bulk` (transforms and products in FP32, state in
+    FP64): L1 < 1e-5 against the closed form at every snapshot."""
+    out = str(tmp_path)
+    write_ic(out, rc.taylor_green_yz(0.0))
+    log = run(os.path.join(CFG, "taylor-green-vortex"), out, "-r", "0", "--fp32-bulk")
+    assert "FP32 bulk" in log
+    snaps = read_all(out, ["vx", "vy", "vz"], (64, 64, 64))
+    assert len(snaps) == 11
+    for hdr, f in snaps:
+        err = rc.l1(f, rc.taylor_green_yz(hdr["t_save"]), ["vx", "vy", "vz"])
+        assert err < 1e-5, (hdr, err)
+
+
 def test_alfven_wave(tmp_path):
     """tests/alfven-wave: L1 < 1e-7 at t_final = 1 (check-alfven-wave.py:38, one period)."""
     out = str(tmp_path)

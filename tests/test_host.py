@@ -110,6 +110,7 @@ def test_command_line(tmp_path):
     assert p["output_dir"] == str(tmp_path / "o")
     p = params(d, "-t", "1", "30", "0")
     assert p["max_walltime_elapsed"] == pytest.approx(0.95 * 1.5)
+    assert p["bulk"] == "f64" and params(d, "--fp32-bulk")["bulk"] == "f32"
     for bad in (["--bogus"], ["-n", "16", "8"], ["-r", "x"]):
         r = subprocess.run([SPOOKY, "--input-dir", d] + bad, capture_output=True, text=True)
         assert r.returncode == 1 and "usage" in r.stderr
