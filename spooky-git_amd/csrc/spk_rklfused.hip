@@ -18,8 +18,12 @@
 
 namespace spk {
 
+// points per thread of the x line (16 in FP32 would widen the column tiles, but
+// its registers spill)
+constexpr int RKLF_E = 8;
+
 template <class T, int N>
-__global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
+__global__ __launch_bounds__(fx_tcols(N, RKLF_E) * fx_tpl(N, RKLF_E)) void k_fwdx_rkl2(
     const cpx<T> *Q, cpx<double> *__restrict__ U, cpx<double> *__restrict__ Uc0, cpx<double> *__restrict__ Uc1,
     cpx<double> *__restrict__ dU0, cpx<T> *__restrict__ W, RklConst c, int stage, double mu, double nu,
     double dtmut, double gdt, int inv, double scale, const cpx<T> *__restrict__ tw,
@@ -44,9 +48,9 @@ __global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
         gdt = cf[3] * dt;
         inv = inv && plan->active[stage + 1];
     }
-    constexpr int E = fx_e(N, 8);
+    constexpr int E = fx_e(N, RKLF_E);
     constexpr int TPL = N / E;
-    constexpr int TC = fx_tcols(N, 8);
+    constexpr int TC = fx_tcols(N, RKLF_E);
     constexpr int LL = lds_len(N);
     constexpr int NT = TC * TPL;
     __shared__ cpx<T> lds[E != N ? TC * LL : 1];
@@ -64,16 +68,21 @@ __global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
     const int j = (int)(col / c.kg.nzh);
     const int kk = (int)(col - (long long)j * c.kg.nzh);
     const long long F = (long long)c.field[0] * N0;
+    double kx0, ky, kz, m0;
+    kvec_at(c.kg, 0, j, kk, kx0, ky, kz, m0);  // ky, kz are constant along the x line
 
     for (int e = tid; e < N; e += NT) twl[e] = tw[e];
     __syncthreads();
 
-    // the three flux components along x, the next one loaded while one is transformed
-    cpx<double> acc[3][E];
+    // k . flux accumulated product by product after each one's x line FFT (the
+    // next product's line is loaded while one is transformed)
+    cpx<double> acc[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) acc[i] = cpx<double>{0.0, 0.0};
     cpx<T> vn[E];
 #pragma unroll
     for (int i = 0; i < E; i++) vn[i] = Q[col + (long long)(t + i * TPL) * ncols];
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 3; q++) {
         cpx<T> v[E];
 #pragma unroll
@@ -84,19 +93,24 @@ __global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
         }
         LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
 #pragma unroll
-        for (int i = 0; i < E; i++) acc[q][i] = cconv<double>(v[i]);
+        for (int i = 0; i < E; i++) {
+            const int n = t + i * TPL;
+            const double kq = q == 0 ? c.kg.kfx * (double)(n < c.kg.nx / 2 ? n : n - c.kg.nx) : (q == 1 ? ky : kz);
+            const cpx<double> a = cconv<double>(v[i]);
+            acc[i] = cpx<double>{acc[i].x + kq * a.x, acc[i].y + kq * a.y};
+        }
     }
 
-    // per point: dtheta = i mask k . flux (DivergenceMask), then rkl2_stage
+    // per point: dtheta = i mask k . flux (DivergenceMask), then rkl2_stage; the
+    // new theta replaces the accumulator
     const double w = 1.0 - mu - nu;
-    cpx<double> th[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
         const int n = t + i * TPL;
         const long long pi = col + (long long)n * ncols;
-        double kx, ky, kz, m;
-        kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
-        const cpx<double> du = mul_plus_im(m, kdot3(kx, acc[0][i], ky, acc[1][i], kz, acc[2][i]));
+        double kx, kyi, kzi, m;
+        kvec_at(c.kg, n, j, kk, kx, kyi, kzi, m);
+        const cpx<double> du = mul_plus_im(m, acc[i]);
         const cpx<double> uc = U[F + pi];
         cpx<double> un;
         if (stage == 1) {
@@ -113,22 +127,23 @@ __global__ __launch_bounds__(fx_tcols(N, 8) * fx_tpl(N, 8)) void k_fwdx_rkl2(
             un = cpx<double>{Y.x + w * u0.x + dtmut * du.x + gdt * d0.x, Y.y + w * u0.y + dtmut * du.y + gdt * d0.y};
         }
         if (live) U[F + pi] = un;
-        th[i] = un;
+        acc[i] = un;
+        // one point's state loads at a time: hoisting all E points' loads would
+        // spill registers
+        __builtin_amdgcn_sched_barrier(0);
     }
 
     if (inv) {
         // i k_d theta of the new theta, 1/N scaled, inverse x pass (k_inv_x with
         // the derivative slots of the RKL2 program) into the next stage's slots
-#pragma unroll
+#pragma unroll 1
         for (int d = 0; d < 3; d++) {
             cpx<T> v[E];
 #pragma unroll
             for (int i = 0; i < E; i++) {
                 const int n = t + i * TPL;
-                double kx, ky, kz, m;
-                kvec_at(c.kg, n, j, kk, kx, ky, kz, m);
-                const double kd = d == 0 ? kx : (d == 1 ? ky : kz);
-                const cpx<double> a = {-(kd * th[i].y), kd * th[i].x};
+                const double kd = d == 0 ? c.kg.kfx * (double)(n < c.kg.nx / 2 ? n : n - c.kg.nx) : (d == 1 ? ky : kz);
+                const cpx<double> a = {-(kd * acc[i].y), kd * acc[i].x};
                 v[i] = cconv<T>(cscale(a, scale));
             }
             LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
@@ -156,9 +171,9 @@ static int launch_t(spk_ctx ctx, int nx, const void *Q, void *U, void *Uc0, void
     const cpx<T> *t = (const cpx<T> *)tw;
 #define SPK_RKLF_CASE(NN)                                                                                      \
     case NN: {                                                                                                 \
-        constexpr int TC = fx_tcols(NN, 8);                                                                    \
+        constexpr int TC = fx_tcols(NN, RKLF_E);                                                          \
         const dim3 grid((unsigned)((ncols + TC - 1) / TC), 1, nboxes(ctx));                                    \
-        k_fwdx_rkl2<T, NN><<<grid, TC * fx_tpl(NN, 8), 0, ctx->stream>>>(q, u, u0, u1, d0, w, c, stage, mu, nu, \
+        k_fwdx_rkl2<T, NN><<<grid, TC * fx_tpl(NN, RKLF_E), 0, ctx->stream>>>(q, u, u0, u1, d0, w, c, stage, mu, nu, \
                                                                           dtmut, gdt, inv, scale, t, plan, ts, bt); \
         break;                                                                                                 \
     }
