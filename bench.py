@@ -24,7 +24,13 @@ Beside the timed line (not part of `value`):
                              boxes at 512x512x2 with RKL2 conduction, FP32 bulk,
                              one batch per shape (spk_ensemble): steps of every box
                              per second, algorithmic GB/s, every box against an
-                             FP64 ensemble and one box per kind against the oracle.
+                             FP64 ensemble and one box per kind against the oracle;
+  closure                    learned subgrid closure, shortened: a 512^3 forced DNS
+                             spun up to t = 0.5, 6 filtered samples, one network
+                             trained per seed (--closure-seeds), then the 128^3 run
+                             with each closure, the plain 128^3 run and the DNS over
+                             0.25 time units: shell-spectrum error up to kc/2 and
+                             wall-time ratio (--no-closure skips it).
 """
 import argparse
 import importlib
@@ -64,6 +70,11 @@ def parse():
     ap.add_argument("--closure-dns", type=int, default=512)
     ap.add_argument("--closure-samples", type=int, default=12)
     ap.add_argument("--closure-sample-dt", type=float, default=0.1)
+    ap.add_argument("--closure-dT", type=float, default=0.5)
+    ap.add_argument("--closure-seeds", default="7,8", help="network initializations, one trained closure each")
+    ap.add_argument("--no-closure", dest="closure", action="store_false",
+                    help="skip the closure leg of the default run (a shortened DNS: spin-up to t = 0.5, "
+                         "6 samples 0.05 apart, 1500 training iterations per seed, 0.25 time units compared)")
     ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure", "closure-kernels"],
                     default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
@@ -516,13 +527,15 @@ def shell_energy(sol):
 
 
 def closure_leg(spk, ini, ctx, torch, N=512, t_spin=1.0, nsamples=6, dt_sample=0.05, dT=0.5, iters=2000,
-                lr=1e-3):
+                lr=1e-3, seeds=(7,)):
     """Learned subgrid closure for forced HD turbulence: a 512^3 FP32-bulk DNS
     (nu = 2e-4, forcing at |k| <= 2, random-phase E(k) ~ k^4 exp(-2 (k/4)^2)
     start) gives filtered velocity / subgrid stress pairs on 128^3; the network is
     trained on them; then from the filtered DNS state the 128^3 run with the
     closure, the plain 128^3 run and the DNS advance by dT, and their shell
-    spectra and wall times are compared (bins 1 .. kc/2, kc = the 128^3 cut)."""
+    spectra and wall times are compared (bins 1 .. kc/2, kc = the 128^3 cut).
+    One network is trained and run per seed (its initial weights); the summary
+    gives the mean and the worst over the seeds."""
     import numpy as np
     T = 2 * np.pi
     n = N // 4
@@ -569,65 +582,71 @@ def closure_leg(spk, ini, ctx, torch, N=512, t_spin=1.0, nsamples=6, dt_sample=0
     tah = [t.download(np.float64, (6, n, n, n + 2))[..., :n] for _, t in samples]
     u_rms = float(np.sqrt(np.mean([np.mean(np.sum(u ** 2, 0)) for u in ubh])))
     tau_rms = [float(np.sqrt(np.mean([np.mean(t[c] ** 2) for t in tah]))) for c in range(6)]
-    cl = spk.Closure(ctx, n)
-    rng = np.random.default_rng(7)
-    p0 = np.zeros(cl.nparams, np.float32)
-    for l, ((ws, bsl), cin) in enumerate(zip(spk.Closure.layer_slices(), spk.Closure.CIN)):
-        w = rng.standard_normal((27, 32, cin)) / np.sqrt(27 * (3 if cin == 16 else cin))
-        if cin == 16:
-            w[..., 3:] = 0
-        if l == 3:
-            w[:, 6:, :] = 0
-            w *= 0.1
-        p0[ws] = w.reshape(-1)
-    cl.set_params(p0)
-    cl.set_scales(1.0 / u_rms, tau_rms)
-    curve = []
-    train_s = 0.0
-    first = None
-    for it in range(iters):
-        rate = lr if it < 0.6 * iters else 0.3 * lr
-        ub, tau = samples[it % (nsamples - 1)]  # the last sample is held out
-        tic = clock()
-        loss = cl.train_step(ub.ptr, tau.ptr, rate, want_loss=(it == 0))
-        train_s += clock() - tic
-        if it == 0:
-            first = loss
-        if (it + 1) % max(1, iters // 8) == 0 or it == iters - 1:
-            tr = float(np.mean([cl.grads(u_.ptr, t_.ptr) for u_, t_ in samples[:-1]]))
-            held = cl.grads(samples[-1][0].ptr, samples[-1][1].ptr)
-            curve.append([it + 1, round(tr, 4), round(held, 4)])
-            print("# closure: iteration %d: training loss %.4f, held-out %.4f" % (it + 1, tr, held),
-                  file=sys.stderr, flush=True)
-    last, held = curve[-1][1], curve[-1][2]
-    res.update(u_rms_filtered=u_rms, tau_rms=tau_rms, train_iters=iters, train_s=round(train_s, 2),
-               train_loss_first=first, train_loss_last=last, heldout_loss=held, loss_curve=curve,
-               ms_per_train_iter=round(1000 * train_s / iters, 3))
-    print("# closure: trained %d iterations in %.1f s, loss %.3f -> %.3f, held-out %.3f" %
-          (iters, train_s, first, last, held), file=sys.stderr, flush=True)
-    # inference time on 128^3
-    dtau = ctx.alloc(6 * fs * 8)
-    cl.forward(samples[-1][0].ptr, dtau.ptr)
-    tic = clock()
-    for _ in range(10):
-        cl.forward(samples[-1][0].ptr, dtau.ptr)
-    res["ms_per_inference"] = round((clock() - tic) * 100, 3)
-    # a posteriori from the filtered DNS state at t0
     les_p = dict(base, nx=n, ny=n, nz=n, t_final=t_end)
     uhat0 = np.stack([np.fft.rfftn(u) for u in ubh[-1]])
     lidx, lval = forced_modes(n)
-    runs = {}
-    for name, with_closure in (("plain", False), ("closure", True)):
+    dtau = ctx.alloc(6 * fs * 8)
+
+    def les_run(cl):
+        """the 128^3 run from the filtered DNS state at t0, with the closure cl or plain"""
         les = spk.Solver(ctx, "f32", **les_p)
         les.set_fields(uhat0)
         les.set_time(t0)
         les.set_forcing(lidx, lval)
         tic = clock()
-        steps = run_until(les, t_end, (lambda s: cl.apply(s)) if with_closure else None)
+        steps = run_until(les, t_end, (lambda s_: cl.apply(s_)) if cl is not None else None)
         wall = clock() - tic
         E = shell_energy(les)
-        runs[name] = dict(steps=steps, wall_s=wall, E=E, finite=bool(np.all(np.isfinite(E))))
         les.destroy()
+        return dict(steps=steps, wall_s=wall, E=E, finite=bool(np.all(np.isfinite(E))))
+
+    runs = {"plain": les_run(None)}
+    per_seed = []
+    for seed in seeds:
+        cl = spk.Closure(ctx, n)
+        rng = np.random.default_rng(seed)
+        p0 = np.zeros(cl.nparams, np.float32)
+        for l, ((ws, bsl), cin) in enumerate(zip(spk.Closure.layer_slices(), spk.Closure.CIN)):
+            w = rng.standard_normal((27, 32, cin)) / np.sqrt(27 * (3 if cin == 16 else cin))
+            if cin == 16:
+                w[..., 3:] = 0
+            if l == 3:
+                w[:, 6:, :] = 0
+                w *= 0.1
+            p0[ws] = w.reshape(-1)
+        cl.set_params(p0)
+        cl.set_scales(1.0 / u_rms, tau_rms)
+        curve = []
+        train_s = 0.0
+        first = None
+        for it in range(iters):
+            rate = lr if it < 0.6 * iters else 0.3 * lr
+            ub, tau = samples[it % (nsamples - 1)]  # the last sample is held out
+            tic = clock()
+            loss = cl.train_step(ub.ptr, tau.ptr, rate, want_loss=(it == 0))
+            train_s += clock() - tic
+            if it == 0:
+                first = loss
+            if (it + 1) % max(1, iters // 4) == 0 or it == iters - 1:
+                tr = float(np.mean([cl.grads(u_.ptr, t_.ptr) for u_, t_ in samples[:-1]]))
+                held = cl.grads(samples[-1][0].ptr, samples[-1][1].ptr)
+                curve.append([it + 1, round(tr, 4), round(held, 4)])
+                print("# closure seed %d: iteration %d: training loss %.4f, held-out %.4f" % (seed, it + 1, tr, held),
+                      file=sys.stderr, flush=True)
+        if seed == seeds[0]:
+            # inference time on 128^3
+            cl.forward(samples[-1][0].ptr, dtau.ptr)
+            tic = clock()
+            for _ in range(10):
+                cl.forward(samples[-1][0].ptr, dtau.ptr)
+            res["ms_per_inference"] = round((clock() - tic) * 100, 3)
+        r = les_run(cl)
+        runs["closure_seed%d" % seed] = r
+        per_seed.append(dict(seed=seed, train_iters=iters, train_s=round(train_s, 2),
+                             ms_per_train_iter=round(1000 * train_s / iters, 3), train_loss_first=first,
+                             train_loss_last=curve[-1][1], heldout_loss=curve[-1][2], loss_curve=curve))
+        cl.destroy()
+    res.update(u_rms_filtered=u_rms, tau_rms=tau_rms, training=per_seed)
     tic = clock()
     dns_steps = run_until(dns, t_end)
     dns_wall = clock() - tic
@@ -649,12 +668,21 @@ def closure_leg(spk, ini, ctx, torch, N=512, t_spin=1.0, nsamples=6, dt_sample=0
         print("# closure: %s %d^3: %d steps %.2f s, spectrum error max %.3f mean %.3f (bins 1..%d)" %
               (name, n, r["steps"], r["wall_s"], float(np.max(err)), float(np.mean(err)), kc // 2),
               file=sys.stderr, flush=True)
+    cls = [out["closure_seed%d" % sd] for sd in seeds]
+    summary = dict(seeds=list(seeds),
+                   closure_max_err_mean_over_seeds=float(np.mean([c["max_rel_spectrum_err"] for c in cls])),
+                   closure_max_err_worst_seed=float(np.max([c["max_rel_spectrum_err"] for c in cls])),
+                   closure_mean_err_mean_over_seeds=float(np.mean([c["mean_rel_spectrum_err"] for c in cls])),
+                   plain_max_err=out["plain"]["max_rel_spectrum_err"],
+                   plain_mean_err=out["plain"]["mean_rel_spectrum_err"],
+                   closure_speedup_vs_dns_min=float(np.min([c["speedup_vs_dns"] for c in cls])),
+                   target="closure run within 10% of the filtered DNS spectrum up to kc/2 at >= 20x less wall time")
+    print("# closure summary: %s" % json.dumps(summary), file=sys.stderr, flush=True)
     res.update(t0=t0, dT=dT, dns_steps_dT=dns_steps, dns_wall_s_dT=round(dns_wall, 2), compare_bins=[1, kc // 2],
-               les=out, E_dns_filtered=[float(x) for x in E_dns[: kc + 1]],
-               E_les_closure=[float(x) for x in runs["closure"]["E"][: kc + 1]],
+               les=out, summary=summary, E_dns_filtered=[float(x) for x in E_dns[: kc + 1]],
+               E_les_closure=[float(x) for x in runs["closure_seed%d" % seeds[0]]["E"][: kc + 1]],
                E_les_plain=[float(x) for x in runs["plain"]["E"][: kc + 1]])
     dns.destroy()
-    cl.destroy()
     return res
 
 
@@ -728,7 +756,8 @@ def main():
     if a.mode == "closure":
         print(json.dumps(closure_leg(spk, ini, ctx, torch, N=a.closure_dns, t_spin=a.closure_spinup,
                                      iters=a.closure_iters, nsamples=a.closure_samples,
-                                     dt_sample=a.closure_sample_dt)))
+                                     dt_sample=a.closure_sample_dt, dT=a.closure_dT,
+                                     seeds=tuple(int(x) for x in a.closure_seeds.split(",")))))
         return
     state = ini.mhd_random((n, n, n), seed=2026)
     if a.mode == "kernels":
@@ -788,6 +817,12 @@ def main():
     if mti:
         print("# mti %d^3: %s" % (n, json.dumps(mti)), file=sys.stderr, flush=True)
     ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split) if a.extra and a.ensemble_steps > 0 else None
+    closure = None
+    if a.extra and a.closure:
+        closure = closure_leg(spk, ini, ctx, torch, N=512, t_spin=0.5, nsamples=6, dt_sample=0.05, dT=0.25,
+                              iters=1500, seeds=tuple(int(x) for x in a.closure_seeds.split(",")))
+        closure["config"] = "DNS 512^3 spun up to t=0.5, 6 samples 0.05 apart, 1500 training iterations per seed, " \
+                            "closure / plain 128^3 / DNS compared after 0.25 time units"
 
     cpu = None
     acc_orc = None
@@ -828,6 +863,7 @@ def main():
         "taylor_green_64": tg,
         "mti_rkl2": mti,
         "ensemble": ens,
+        "closure": closure,
         "fft_pass_gbs_f64": {k: round(v["gbs"], 1) for k, v in passes.items()},
         "kernels_f64": {k: {"ms": round(v["ms"], 4), "gbs": round(v["gbs"], 1), "share": round(v["share"], 4)}
                         for k, v in kern.items()},
