@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace spk {
 
 template <class T>
@@ -384,11 +386,16 @@ struct Batch {
     long long off[BATCH_ARGS];
 };
 
-// argument i of a batched launch for this workgroup's box
+// argument i of a batched launch for this workgroup's box.  The address read
+// from the table is marked as global memory: a pointer loaded from memory is
+// otherwise generic, and accesses through it would be flat loads and stores.
 template <class P>
 __device__ __forceinline__ P rebase(const Batch &b, int i, P p) {
     if (b.rows == nullptr || b.slot[i] < 0) return p;
-    return reinterpret_cast<P>(b.rows[blockIdx.z].p[b.slot[i]] + b.off[i]);
+    using E = __attribute__((address_space(1))) std::remove_pointer_t<P>;
+    using GC = __attribute__((address_space(1))) char;
+    GC *g = (GC *)(b.rows[blockIdx.z].p[b.slot[i]]) + b.off[i];
+    return (P)(E *)g;
 }
 __device__ __forceinline__ const BoxRow *batch_row(const Batch &b) {
     return b.rows == nullptr ? nullptr : b.rows + blockIdx.z;

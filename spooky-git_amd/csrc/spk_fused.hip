@@ -1,73 +1,22 @@
-// spk_fused.hip -- instantiations and launcher of the fused forward x pass +
-// spectral epilogue + RK3 update kernel (k_fwdx_epilogue).
-#include "spk_internal.hpp"
-#include "spk_kernels.hpp"
+// spk_fused.hip -- the launcher of the fused forward x pass + spectral
+// epilogue + RK3 update kernel (k_fwdx_epilogue): picks the instantiation set
+// (precision; batched launch or not) compiled in spk_fused_*.hip.
+#include "spk_fused_launch.hpp"
 
 namespace spk {
 
 bool fused_supported_nx(int nx) { return nx >= 32 && nx <= 1024 && is_pow2(nx); }
-
-template <class T, int N, int LY, int EF>
-static void launch_ly_mode(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T> *P, cpx<double> *U,
-                           cpx<T> *S, cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts,
-                           const Batch &bt) {
-    switch (mode) {
-        case 1: k_fwdx_epilogue<T, N, LY, 1, EF><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
-        case 2: k_fwdx_epilogue<T, N, LY, 2, EF><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
-        default: k_fwdx_epilogue<T, N, LY, 3, EF><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
-    }
-}
-
-template <class T, int N, int EF>
-static int launch_n(int ly, int mode, hipStream_t st, const void *P, void *U, void *S, void *W, const FusedConst &fc,
-                    const void *tw, const TimeState *ts, int ngroups, const Batch &bt, unsigned nbz) {
-    constexpr int TC = fx_tcols(N, EF);
-    const long long ncols = (long long)fc.e.kg.ny * fc.e.kg.nzh;
-    dim3 grid((unsigned)((ncols + TC - 1) / TC), ngroups, nbz);
-    const int bs = TC * fx_tpl(N, EF);
-    const cpx<T> *p = (const cpx<T> *)P;
-    cpx<double> *u = (cpx<double> *)U;
-    cpx<T> *s = (cpx<T> *)S;  // the RK3 increment register, bulk precision
-    cpx<T> *w = (cpx<T> *)W;
-    const cpx<T> *t = (const cpx<T> *)tw;
-    switch (ly) {
-        case LY_HD: launch_ly_mode<T, N, LY_HD, EF>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        case LY_HD_TH: launch_ly_mode<T, N, LY_HD_TH, EF>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        case LY_MHD: launch_ly_mode<T, N, LY_MHD, EF>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        case LY_MHD_TH: launch_ly_mode<T, N, LY_MHD_TH, EF>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        default: set_error("fused epilogue: layout %d has no FFT products", ly); return SPK_ERR_STATE;
-    }
-    SPK_LAUNCH_CHECK();
-    return SPK_OK;
-}
-
-template <class T, int EF>
-static int launch_t(int nx, int ly, int mode, hipStream_t st, const void *P, void *U, void *S, void *W,
-                    const FusedConst &fc, const void *tw, const TimeState *ts, int ngroups, const Batch &bt,
-                    unsigned nbz) {
-    switch (nx) {
-        case 32: return launch_n<T, 32, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        case 64: return launch_n<T, 64, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        case 128: return launch_n<T, 128, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        case 256: return launch_n<T, 256, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        case 512: return launch_n<T, 512, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        case 1024: return launch_n<T, 1024, EF>(ly, mode, st, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        default: set_error("fused epilogue: NX=%d not instantiated", nx); return SPK_ERR_ARG;
-    }
-}
 
 // ef: points per thread of the x line, 4 or 8 (picked by the solver's tile autotune)
 int launch_fwdx_epilogue(spk_ctx ctx, int bulk, int nx, int ly, int mode, int ef, const void *P, void *U, void *S,
                          void *W, const FusedConst &fc, const void *tw, const TimeState *ts, int ngroups) {
     Batch bt;
     SPK_TRY(make_batch(ctx, {P, U, S, W, ts}, &bt));
-    const unsigned nbz = nboxes(ctx);
-    if (bulk == SPK_F64) {
-        if (ef == 8) return launch_t<double, 8>(nx, ly, mode, ctx->stream, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-        return launch_t<double, 4>(nx, ly, mode, ctx->stream, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-    }
-    if (ef == 8) return launch_t<float, 8>(nx, ly, mode, ctx->stream, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
-    return launch_t<float, 4>(nx, ly, mode, ctx->stream, P, U, S, W, fc, tw, ts, ngroups, bt, nbz);
+    if (bulk == SPK_F64)
+        return bt.rows ? launch_fused_f64_batched(ctx, nx, ly, mode, ef, P, U, S, W, fc, tw, ts, ngroups, bt)
+                       : launch_fused_f64(ctx, nx, ly, mode, ef, P, U, S, W, fc, tw, ts, ngroups, bt);
+    return bt.rows ? launch_fused_f32_batched(ctx, nx, ly, mode, ef, P, U, S, W, fc, tw, ts, ngroups, bt)
+                   : launch_fused_f32(ctx, nx, ly, mode, ef, P, U, S, W, fc, tw, ts, ngroups, bt);
 }
 
 }  // namespace spk

@@ -232,18 +232,22 @@ __host__ __device__ constexpr int zmid_rw(int m) {
     return (m / zmid_e<T>(m)) >= 256 ? 1 : 256 / (m / zmid_e<T>(m));
 }
 
-template <class T, int NZ, int PROG>
+// BAT: a batched launch (blockIdx.z is the box); single-box launches use the
+// BAT = false instantiation, whose code has no table lookups
+template <class T, int NZ, int PROG, bool BAT = false>
 __global__ __launch_bounds__(256) void k_zmid(const cpx<T> *__restrict__ W, const cpx<T> *__restrict__ Wr,
                                               cpx<T> *__restrict__ P, long long nrows, long long fs,
                                               const cpx<T> *__restrict__ tw_nz,
                                               const cpx<T> *__restrict__ tw_m, ZConst zc,
                                               double *__restrict__ pmax, const int *__restrict__ gate, Batch bt) {
-    W = rebase(bt, 0, W);
-    Wr = rebase(bt, 1, Wr);
-    P = rebase(bt, 2, P);
-    pmax = rebase(bt, 3, pmax);
-    gate = rebase(bt, 4, gate);
-    if (const BoxRow *br = batch_row(bt)) zc.chi = 1.0 / br->reynolds_ani;
+    if constexpr (BAT) {
+        W = rebase(bt, 0, W);
+        Wr = rebase(bt, 1, Wr);
+        P = rebase(bt, 2, P);
+        pmax = rebase(bt, 3, pmax);
+        gate = rebase(bt, 4, gate);
+        zc.chi = 1.0 / batch_row(bt)->reynolds_ani;
+    }
     if (gate != nullptr && *gate == 0) return;
     using Z = ZProg<PROG>;
     constexpr int M = NZ / 2;
@@ -796,17 +800,18 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     }
 }
 
-template <class T, int N, int LY, int MODE, int EF>
+template <class T, int N, int LY, int MODE, int EF, bool BAT = false>
 __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
     const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ S, cpx<T> *__restrict__ W,
     FusedConst fc, const cpx<T> *__restrict__ tw, const TimeState *__restrict__ ts, Batch bt) {
     using L = Layout<LY>;
-    P = rebase(bt, 0, P);
-    U = rebase(bt, 1, U);
-    S = rebase(bt, 2, S);
-    W = rebase(bt, 3, W);
-    ts = rebase(bt, 4, ts);
-    if (const BoxRow *br = batch_row(bt)) {
+    if constexpr (BAT) {
+        P = rebase(bt, 0, P);
+        U = rebase(bt, 1, U);
+        S = rebase(bt, 2, S);
+        W = rebase(bt, 3, W);
+        ts = rebase(bt, 4, ts);
+        const BoxRow *br = batch_row(bt);
         fc.e.nu = br->nu;
         fc.e.nu_m = br->nu_m;
         fc.e.nu_th = br->nu_th;

@@ -748,6 +748,29 @@ int launch_inv_x(spk_solver s, const InvProg &prog, int nslots, void *W, double 
     return SPK_OK;
 }
 
+template <class T, int NZ, bool BAT>
+int launch_zmid_prog(int prog, dim3 grid, int nt, hipStream_t st, const cpx<T> *w, const cpx<T> *wr, cpx<T> *pp,
+                     long long nrows, long long ntc, const cpx<T> *tz, const cpx<T> *tm, const ZConst &zc,
+                     double *pmax, const int *gate, const Batch &bt) {
+#define SPK_ZP(PR)                                                                                           \
+    case PR:                                                                                                 \
+        k_zmid<T, NZ, PR, BAT><<<grid, nt, 0, st>>>(w, wr, pp, nrows, ntc, tz, tm, zc, pmax, gate, bt); \
+        break;
+    switch (prog) {
+        SPK_ZP(ZP_U)
+        SPK_ZP(ZP_U | ZP_B)
+        SPK_ZP(ZP_U | ZP_TH)
+        SPK_ZP(ZP_U | ZP_B | ZP_TH)
+        SPK_ZP(ZP_U | ZP_B | ZP_TH | ZP_G)
+        SPK_ZP(ZP_B | ZP_G)
+        SPK_ZP(ZP_B | ZP_G | ZP_BR)
+        default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
+    }
+#undef SPK_ZP
+    SPK_LAUNCH_CHECK();
+    return SPK_OK;
+}
+
 template <class T, int NZ>
 int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax, unsigned *nblocks_out) {
     constexpr int M = NZ / 2;
@@ -761,28 +784,14 @@ int launch_zmid_nz(spk_solver s, int prog, const void *W, void *P, double *pmax,
     const cpx<T> *w = (const cpx<T> *)W;
     const cpx<T> *wr = (const cpx<T> *)s->Breal;
     cpx<T> *pp = (cpx<T> *)P;
-    hipStream_t st = s->ctx->stream;
     Batch bt;
     SPK_TRY(make_batch(s->ctx, {W, s->Breal, P, pmax, s->ctx->gate}, &bt));
     const dim3 grid(nb, 1, nboxes(s->ctx));
-    switch (prog) {
-        case ZP_U: k_zmid<T, NZ, ZP_U><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
-        case ZP_U | ZP_B: k_zmid<T, NZ, ZP_U | ZP_B><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
-        case ZP_U | ZP_TH: k_zmid<T, NZ, ZP_U | ZP_TH><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
-        case ZP_U | ZP_B | ZP_TH:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
-            break;
-        case ZP_U | ZP_B | ZP_TH | ZP_G:
-            k_zmid<T, NZ, ZP_U | ZP_B | ZP_TH | ZP_G><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
-            break;
-        case ZP_B | ZP_G: k_zmid<T, NZ, ZP_B | ZP_G><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt); break;
-        case ZP_B | ZP_G | ZP_BR:
-            k_zmid<T, NZ, ZP_B | ZP_G | ZP_BR><<<grid, NT, 0, st>>>(w, wr, pp, nrows, s->g.ntc, tz, tm, zc, pmax, s->ctx->gate, bt);
-            break;
-        default: set_error("fused z program %d not instantiated", prog); return SPK_ERR_STATE;
-    }
-    SPK_LAUNCH_CHECK();
-    return SPK_OK;
+    if (bt.rows)
+        return launch_zmid_prog<T, NZ, true>(prog, grid, NT, s->ctx->stream, w, wr, pp, nrows, s->g.ntc, tz, tm, zc,
+                                             pmax, s->ctx->gate, bt);
+    return launch_zmid_prog<T, NZ, false>(prog, grid, NT, s->ctx->stream, w, wr, pp, nrows, s->g.ntc, tz, tm, zc,
+                                          pmax, s->ctx->gate, bt);
 }
 
 template <class T>
