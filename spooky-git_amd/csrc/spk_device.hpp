@@ -175,6 +175,26 @@ __host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * i
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_len(int n) { return n + n / 16 + 1; }
 
+// LDS placement policies of a line transform: at(i) is the slot of element i,
+// len(n) the slots a line of n elements takes.
+struct PadSkew {  // one pad slot every 16 elements
+    __device__ static __forceinline__ int at(int i) { return lpad(i); }
+    __host__ __device__ static constexpr int len(int n) { return lds_len(n); }
+};
+// No pad: the 16 elements of group g = i / 16 are permuted within the group,
+// slot i ^ s(g) with s(g) = 3 (g + (g & 1)) mod 16.  For the access patterns
+// of the z rows (several rows of 16-128 points per wavefront, natural-order
+// stores and loads, the mirrored loads of the real<->complex split and the
+// radix-4 exchanges) this leaves about a quarter of the bank-conflict cycles
+// of the skewed layout (64-bank model of 16-byte and 8-byte accesses).
+struct PadXor {
+    __device__ static __forceinline__ int at(int i) {
+        const int g = i >> 4;
+        return i ^ ((3 * (g + (g & 1))) & 15);
+    }
+    __host__ __device__ static constexpr int len(int n) { return (n + 15) & ~15; }
+};
+
 // Twiddle table convention: tw[e] = exp(-2 pi i e / N), e = 0..N-1 (forward).
 template <int DIR, class T>
 __device__ __forceinline__ cpx<T> tw_load(const cpx<T>* __restrict__ tw, int e) {
@@ -218,7 +238,7 @@ struct LineTw {
     cpx<T> wt[NW];
 };
 
-template <class T, int N, int E, int DIR, bool WAVE = false, int TWM = 0>
+template <class T, int N, int E, int DIR, bool WAVE = false, int TWM = 0, class PAD = PadSkew>
 struct LineFFT {
     static_assert(N % E == 0, "E must divide N");
     static constexpr int TPL = N / E;                  // threads per line
@@ -296,10 +316,10 @@ struct LineFFT {
             const int jm = t % Ns;
             const int idxD = (t / Ns) * Ns * E + jm;
 #pragma unroll
-            for (int r = 0; r < E; r++) lds[lpad(idxD + r * Ns)] = v[r];
+            for (int r = 0; r < E; r++) lds[PAD::at(idxD + r * Ns)] = v[r];
             lsync<WAVE>();
 #pragma unroll
-            for (int i = 0; i < E; i++) v[i] = lds[lpad(t + i * TPL)];
+            for (int i = 0; i < E; i++) v[i] = lds[PAD::at(t + i * TPL)];
             lsync<WAVE>();
         }
         if constexpr (P + 1 < NPASS) pass_full<P + 1>(v, lds, t, tw, wn, tr);

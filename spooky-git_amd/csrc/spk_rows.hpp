@@ -22,11 +22,12 @@ namespace spk {
 template <class T, int M, int E, bool REG = false>
 struct ZRow {
     static constexpr int TPL = M / E;
-    static constexpr int LL = lds_len(M + 1);  // LDS elements per row
+    using PAD = PadXor;
+    static constexpr int LL = PAD::len(M + 1);  // LDS elements per row
     // the threads of a row share one wavefront: wavefront-level ordering only
     static constexpr bool WAVE = TPL <= 64;
-    using FFTI = LineFFT<T, M, E, +1, WAVE, REG ? 2 : 0>;
-    using FFTF = LineFFT<T, M, E, -1, WAVE, REG ? 2 : 0>;
+    using FFTI = LineFFT<T, M, E, +1, WAVE, REG ? 2 : 0, PAD>;
+    using FFTF = LineFFT<T, M, E, -1, WAVE, REG ? 2 : 0, PAD>;
 
     // a thread's twiddles: the length-M FFT's and W^k = exp(-2 pi i k / NZ) of
     // its points k = t + i*TPL for the split pre/post-processing
@@ -83,20 +84,20 @@ struct ZRow {
                 const int k = t + i * TPL;
                 cpx<T> a = cscale(v[i], scale);
                 if (k == 0) a.y = (T)0;
-                L[lpad(k)] = a;
+                L[PAD::at(k)] = a;
                 v[i] = a;  // X[k] stays in registers; only X[M-k] comes back through LDS
             }
             if (t == 0) {
                 cpx<T> a = cscale(nyq, scale);
                 a.y = (T)0;
-                L[lpad(M)] = a;
+                L[PAD::at(M)] = a;
             }
         }
         lsync<WAVE>();
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const int k = t + i * TPL;
-            const cpx<T> xk = v[i], xm = L[lpad(M - k)];
+            const cpx<T> xk = v[i], xm = L[PAD::at(M - k)];
             const cpx<T> s = {xk.x + xm.x, xk.y - xm.y};
             const cpx<T> d = {xk.x - xm.x, xk.y + xm.y};
             cpx<T> w;
@@ -132,14 +133,14 @@ struct ZRow {
         if constexpr (REG) FFTF::run_tw(v, L, t, rt->f);
         else FFTF::run(v, L, t, tw_m);
 #pragma unroll
-        for (int i = 0; i < E; i++) L[lpad(t + i * TPL)] = v[i];
+        for (int i = 0; i < E; i++) L[PAD::at(t + i * TPL)] = v[i];
         lsync<WAVE>();
         if (active) {
             const T h = (T)0.5;
 #pragma unroll
             for (int i = 0; i < E; i++) {
                 const int k = t + i * TPL;
-                const cpx<T> zk = v[i], zm = L[lpad((M - k) & (M - 1))];
+                const cpx<T> zk = v[i], zm = L[PAD::at((M - k) & (M - 1))];
                 const cpx<T> ev = {h * (zk.x + zm.x), h * (zk.y - zm.y)};
                 const cpx<T> od = {h * (zk.y + zm.y), -h * (zk.x - zm.x)};
                 cpx<T> w;
@@ -148,7 +149,7 @@ struct ZRow {
                 dst[k] = cconv<TO>(ev + cmul(od, w));
             }
             if (t == 0) {
-                const cpx<T> z0 = L[0];
+                const cpx<T> z0 = L[PAD::at(0)];
                 dst[M] = cconv<TO>(cpx<T>{z0.x - z0.y, (T)0});
             }
         }
