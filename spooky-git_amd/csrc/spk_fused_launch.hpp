@@ -9,15 +9,33 @@
 
 namespace spk {
 
-template <class T, int N, int LY, int EF, bool BAT>
+template <class T, int N, int LY, int EF, bool BAT, bool STRAT = false>
 static void launch_ly_mode(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T> *P, cpx<double> *U,
                            cpx<T> *S, cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts,
                            const Batch &bt) {
     switch (mode) {
-        case 1: k_fwdx_epilogue<T, N, LY, 1, EF, BAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
-        case 2: k_fwdx_epilogue<T, N, LY, 2, EF, BAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
-        default: k_fwdx_epilogue<T, N, LY, 3, EF, BAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
+        case 1: k_fwdx_epilogue<T, N, LY, 1, EF, BAT, STRAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
+        case 2: k_fwdx_epilogue<T, N, LY, 2, EF, BAT, STRAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
+        default: k_fwdx_epilogue<T, N, LY, 3, EF, BAT, STRAT><<<grid, bs, 0, st>>>(P, U, S, W, fc, tw, ts, bt); break;
     }
+}
+
+// Boussinesq layouts: stratified runs take the velocity + temperature group,
+// instantiated with 4 points per thread only (with 8 its 4 accumulated
+// components spill registers)
+template <class T, int N, int LY, int EF, bool BAT>
+static int launch_bouss(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T> *P, cpx<double> *U, cpx<T> *S,
+                        cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts, const Batch &bt) {
+    if (!fc.e.stratification) {
+        launch_ly_mode<T, N, LY, EF, BAT>(mode, grid, bs, st, P, U, S, W, fc, tw, ts, bt);
+        return SPK_OK;
+    }
+    if constexpr (EF == 4) {
+        launch_ly_mode<T, N, LY, EF, BAT, true>(mode, grid, bs, st, P, U, S, W, fc, tw, ts, bt);
+        return SPK_OK;
+    }
+    set_error("fused epilogue: stratified runs use 4 points per thread");
+    return SPK_ERR_ARG;
 }
 
 template <class T, int N, int EF, bool BAT>
@@ -34,9 +52,9 @@ static int launch_n(int ly, int mode, hipStream_t st, const void *P, void *U, vo
     const cpx<T> *t = (const cpx<T> *)tw;
     switch (ly) {
         case LY_HD: launch_ly_mode<T, N, LY_HD, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        case LY_HD_TH: launch_ly_mode<T, N, LY_HD_TH, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
+        case LY_HD_TH: SPK_TRY((launch_bouss<T, N, LY_HD_TH, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt))); break;
         case LY_MHD: launch_ly_mode<T, N, LY_MHD, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
-        case LY_MHD_TH: launch_ly_mode<T, N, LY_MHD_TH, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt); break;
+        case LY_MHD_TH: SPK_TRY((launch_bouss<T, N, LY_MHD_TH, EF, BAT>(mode, grid, bs, st, p, u, s, w, fc, t, ts, bt))); break;
         default: set_error("fused epilogue: layout %d has no FFT products", ly); return SPK_ERR_STATE;
     }
     SPK_LAUNCH_CHECK();

@@ -688,7 +688,7 @@ struct spk_solver_s {
     int force_n;               // large-scale forcing: modes, their state indices and values (device)
     long long *force_idx;
     cpx<double> *force_val;
-    InvProg inv_rkl_b, inv_rkl_g;
+    InvProg inv_rkl_b, inv_rkl_g, inv_th;
     cpx<double> *rUc0, *rUc1, *rdU0;
     TimeState *ts;
     double *pmax, *part, *res, *spec;
@@ -698,6 +698,7 @@ struct spk_solver_s {
     int step;
     double tremap;
     bool loop_started, products_valid;
+    bool fused_w_ok;  // W holds the inverse x pass of the state after the last fused step
     const void *products_src;  // the state whose RHS products P holds when products_valid
     int fused_mode;      // caller's switch for k_fwdx_epilogue: 0 never, 1 autotuned, 2 always
     bool fused_faster;   // tile autotune: the fused back end beat the unfused one on this grid
@@ -899,20 +900,23 @@ int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
 // ---- fused path: forward y pass, then forward x + epilogue + RK3 (+ projection
 // + next inverse x) in one kernel; the next stage starts at the inverse y pass ----
 
-// Each field group of k_fwdx_epilogue updates its own fields in place, so the
-// groups must not read each other's state: stratification (u_s <-> theta) is
-// excluded, as are shearing (sheared k between stages, remap), sts and the
-// in-RHS conduction (its grad theta slots come from k_inv_x).
+// Each field group of k_fwdx_epilogue updates its own fields in place; with
+// stratification (u_s <-> theta) the velocity and temperature groups share their
+// blocks.  Shearing (sheared k between stages, remap) and the in-RHS conduction
+// (its grad theta slots come from k_inv_x) stay on the unfused back end.
 bool fused_allowed(spk_solver s) {
-    return s->nin > 0 && !s->aniso_rhs && !s->p.shearing && s->nsts == 0 &&
-           !(s->p.boussinesq && s->p.stratification) && fused_supported_nx(s->p.nx) && s->ly != LY_HEAT;
+    return s->nin > 0 && !s->aniso_rhs && !s->p.shearing && fused_supported_nx(s->p.nx) && s->ly != LY_HEAT;
 }
 
 bool fused_ok(spk_solver s) {
     return fused_allowed(s) && (s->fused_mode == 2 || (s->fused_mode == 1 && s->fused_faster));
 }
 
-int fused_groups(spk_solver s) { return 1 + (s->p.mhd ? 1 : 0) + (s->p.boussinesq ? 1 : 0); }
+// grid rows of k_fwdx_epilogue: velocity (+ temperature when stratified),
+// magnetic, temperature
+int fused_groups(spk_solver s) {
+    return 1 + (s->p.mhd ? 1 : 0) + (s->p.boussinesq && !s->p.stratification ? 1 : 0);
+}
 
 int rhs_back_fused(spk_solver s, int mode, double gam, double xi, int project, int inv) {
     const Grid3 &g = s->g;
@@ -1307,7 +1311,11 @@ int tune_fused(spk_solver s) {
     };
     float best = 1e30f, t_unf = 0.f, t_cl = 0.f;
     int rc = SPK_OK;
+    // stratified runs: 4 points per thread only (k_fwdx_epilogue's velocity +
+    // temperature group)
+    const bool strat = s->p.boussinesq && s->p.stratification;
     for (int ef : {4, 8}) {
+        if (strat && ef != 4) continue;
         float ms = 0.f;
         rc = time_it([&] {
             return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, 2, ef, s->P, s->U, s->S, s->W, fc, tw, s->ts,
@@ -1337,24 +1345,46 @@ int tune_fused(spk_solver s) {
     return SPK_OK;
 }
 
-// One main-loop iteration on the fused path.  w_valid: W holds the inverse x
-// pass of the current, already projected state (written by the previous
-// step's stage 3); otherwise CleanFieldDivergence and the inverse x pass run first.
+// One main-loop iteration on the fused path (HydroMHDAdvance,
+// hydro_mhd_advance.cu:18-75).  w_valid: W holds the inverse x pass of the
+// current, already projected state (written by the previous step's stage 3);
+// otherwise CleanFieldDivergence and the inverse x pass run first.  The RKL2
+// cycle of a super-time-stepped run sits where the unfused path has it: on odd
+// steps after compute_dt, whose real fields (here the stage-1 products) it
+// leaves as they were, and on even steps after RK3, after which the changed
+// fields' inverse x pass in W is made again (theta alone) or left to the next
+// step (s->fused_w_ok).
 int step_fused(spk_solver s, bool w_valid) {
     if (!w_valid) {
         SPK_TRY(clean_divergence(s));
         if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv, s->nin, s->W, 0.0));
         else SPK_TRY(launch_inv_x<float>(s, s->inv, s->nin, s->W, 0.0));
     }
+    const bool sts = s->p.supertimestepping && s->nsts > 0;
+    auto cycle = [&]() { return s->rkl_device ? rkl2_cycle_device(s) : rkl2_cycle(s); };
     // compute_dt: the stage-1 transforms carry the CFL maxima
     SPK_TRY(rhs_front_from_w(s, true));
     SPK_TRY(dt_kernel(s));
+    if (sts && (s->step % 2) == 1) SPK_TRY(cycle());
     SPK_TRY(rhs_back_fused(s, 1, gammaRK[0], xiRK[0], 0, 1));
     SPK_TRY(rhs_front_from_w(s, false));
     SPK_TRY(rhs_back_fused(s, 2, gammaRK[1], xiRK[1], 0, 1));
     SPK_TRY(rhs_front_from_w(s, false));
     SPK_TRY(rhs_back_fused(s, 3, gammaRK[2], 0.0, 1, 1));
     SPK_TRY(clock(s, 1, 0, 0, 0, 0.0));
+    s->fused_w_ok = true;
+    if (sts && (s->step % 2) == 0) {
+        SPK_TRY(cycle());
+        if (s->nsts == 1 && s->sts_index[0] == s->TH) {
+            // theta's W slot is its field index (the RHS program's slot order)
+            const Grid3 &g = s->g;
+            void *wt = (char *)s->W + (size_t)s->TH * (size_t)g.ntc * esize(s->bulk);
+            if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_th, 1, wt, 0.0));
+            else SPK_TRY(launch_inv_x<float>(s, s->inv_th, 1, wt, 0.0));
+        } else {
+            s->fused_w_ok = false;  // u or B changed: project and transform at the next step
+        }
+    }
     s->step += 1;
     s->products_valid = false;
     return clock(s, 0, 1, 0, 0, 0.0);
@@ -1480,6 +1510,8 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     for (int q = 0; q < 3; q++) s->inv_rkl.s[3 + q] = InvSlot{s->TH, q};
     memset(&s->inv_rkl_b, 0, sizeof(s->inv_rkl_b));
     memset(&s->inv_rkl_g, 0, sizeof(s->inv_rkl_g));
+    memset(&s->inv_th, 0, sizeof(s->inv_th));
+    s->inv_th.s[0] = InvSlot{s->TH, -1};
     for (int q = 0; q < 3; q++) s->inv_rkl_b.s[q] = InvSlot{s->BX + q, -1};
     for (int q = 0; q < 3; q++) s->inv_rkl_g.s[q] = InvSlot{s->TH, q};
     memset(&s->rkl, 0, sizeof(s->rkl));
@@ -1689,7 +1721,7 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
         if (fused) {
             // forcing changes the state after the fused back end made the next step's
             // inverse x pass, so that pass is made again from the state
-            SPK_TRY(step_fused(s, n > 0 && s->force_n == 0));
+            SPK_TRY(step_fused(s, n > 0 && s->force_n == 0 && s->fused_w_ok));
         } else {
             SPK_TRY(clean_divergence(s));
             s->products_valid = false;

@@ -64,6 +64,10 @@ CASES = {
     "hd_strat_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, stratification=1,
                                      strat_direction=1, N2=0.5, nu=0.01, nu_th=0.01, cfl=0.5, lx=1.0, ly=1.0,
                                      lz=1.0, **BIG_T), fields="ut"),
+    "hd_strat_sts_u_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, stratification=1,
+                                           strat_direction=1, N2=0.5, nu=0.02, nu_th=0.01, cfl=0.5,
+                                           supertimestepping=1, sts_mask=0b111, safety_sts=10, cfl_par=1.0,
+                                           lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
     "hd_bouss_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, nu=0.01, nu_th=0.01,
                                      cfl=0.5, lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
     "mhd_bouss_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1, nu=0.01,
@@ -98,5 +102,9 @@ def relerr(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
-# cases on which spk_main_loop takes the fused path (32 <= NX <= 1024, no shearing/sts/conduction)
-FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d"]
+# cases on which spk_main_loop takes the fused path (32 <= NX <= 1024, no shearing or in-RHS
+# conduction): the stratified ones run velocity and temperature as one group of the fused kernel,
+# the super-time-stepped ones interleave RKL2 cycles (theta alone: its inverse x pass is made again
+# after an even step's cycle; velocity: the next step starts from a fresh projection and transform)
+FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d", "mhd_strat_x32", "hd_strat_x32",
+               "mhd_bouss_rkl2_x32", "hd_strat_sts_u_x32"]
