@@ -53,10 +53,66 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 
 constexpr size_t conv_lds(int cin) { return (size_t)(NTAP * CH * cin + HX * HY * HZ * cin) * sizeof(bf16); }
 
+constexpr int CONV_T = 512;  // 8 waves: two per SIMD, so one wave's LDS reads overlap the other's MFMAs
+
+// One brick's input with its one-point halo (HX x HY x HZ points of CIN
+// channels, channels-last, periodic wrap), held in registers between its global
+// loads (fetch) and its LDS stores (stash: channels-last, chunk-swizzled by halo
+// point; stash_t: channel-major rows of the weight-gradient layout), so that a
+// block loads brick b + 1 while it computes brick b.
+template <int CIN>
+struct HaloPrefetch {
+    static constexpr int NQ = CIN / 8;
+    static constexpr int NCH = HX * HY * HZ * NQ;  // 16-byte chunks
+    static constexpr int NPF = (NCH + CONV_T - 1) / CONV_T;
+    bf16x8 v[NPF];
+
+    __device__ __forceinline__ void fetch(const bf16 *__restrict__ X, int b, int n, int tid) {
+        const int nbz = n / 32, nby = n / 4;
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
+#pragma unroll
+        for (int k = 0; k < NPF; k++) {
+            const int i = tid + k * CONV_T;
+            if (i < NCH) {
+                const int q = i % NQ, hp = i / NQ;
+                const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
+                const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
+                v[k] = *reinterpret_cast<const bf16x8 *>(X + (((size_t)gx * n + gy) * n + gz) * CIN + 8 * q);
+            }
+        }
+    }
+    __device__ __forceinline__ void stash(bf16 *Xs, int tid) const {
+#pragma unroll
+        for (int k = 0; k < NPF; k++) {
+            const int i = tid + k * CONV_T;
+            if (i < NCH) {
+                const int q = i % NQ, hp = i / NQ;
+                *reinterpret_cast<bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN)) = v[k];
+            }
+        }
+    }
+    // Xs[ci][XPL]: halo point (hx, hy, hz) at (hx HY + hy) 48 + 7 + hz
+    template <int XPLEN>
+    __device__ __forceinline__ void stash_t_impl(bf16 *Xs, int tid) const {
+#pragma unroll
+        for (int k = 0; k < NPF; k++) {
+            const int i = tid + k * CONV_T;
+            if (i < NCH) {
+                const int q = i % NQ, hp = i / NQ;
+                const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
+                const int e = (hx * HY + hy) * 48 + 7 + hz;
+#pragma unroll
+                for (int j = 0; j < 8; j++) Xs[(8 * q + j) * XPLEN + e] = v[k][j];
+            }
+        }
+    }
+    __device__ __forceinline__ void stash_t(bf16 *Xs, int tid) const;
+};
+
 // MODE 0: hidden layer, Y = GELU(conv + b); 1: also Z = conv + b (training);
 // 2: output layer, out[c][p] = conv + b for c < 6 (FP32, channel-major);
 // 3: input gradient, Y = conv(dZ, flipped transposed W) * GELU'(Zp)
-constexpr int CONV_T = 512;  // 8 waves: two per SIMD, so one wave's LDS reads overlap the other's MFMAs
 
 template <int CIN, int MODE>
 __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
@@ -79,19 +135,16 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
     const int nbz = n / 32, nby = n / 4;
     const int nb = (n / 4) * nby * nbz;
     const size_t P = (size_t)n * n * n;
+    HaloPrefetch<CIN> hf;
+    if (blockIdx.x < nb) hf.fetch(X, blockIdx.x, n, tid);
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
         __syncthreads();
-        for (int i = tid; i < HX * HY * HZ * NQ; i += CONV_T) {
-            const int q = i % NQ, hp = i / NQ;
-            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
-            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
-            const size_t g = (((size_t)gx * n + gy) * n + gz) * CIN + 8 * q;
-            *reinterpret_cast<bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN)) =
-                *reinterpret_cast<const bf16x8 *>(X + g);
-        }
+        hf.stash(Xs, tid);
         __syncthreads();
+        // the next brick's loads are in flight during this brick's matrix work
+        if (b + (int)gridDim.x < nb) hf.fetch(X, b + gridDim.x, n, tid);
         f32x16 acc[2];
 #pragma unroll
         for (int ys = 0; ys < 2; ys++)
@@ -172,18 +225,15 @@ __global__ __launch_bounds__(CONV_T) void k_conv_out(const bf16 *__restrict__ X,
     const int nbz = n / 32, nby = n / 4;
     const int nb = (n / 4) * nby * nbz;
     const size_t P = (size_t)n * n * n;
+    HaloPrefetch<CH> hf;
+    if (blockIdx.x < nb) hf.fetch(X, blockIdx.x, n, tid);
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
         __syncthreads();
-        for (int i = tid; i < HX * HY * HZ * 4; i += CONV_T) {
-            const int qq = i & 3, hp = i >> 2;
-            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
-            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
-            const size_t g = (((size_t)gx * n + gy) * n + gz) * CH + 8 * qq;
-            *reinterpret_cast<bf16x8 *>(Xs + hp * CH + 8 * swz(qq, hp, CH)) = *reinterpret_cast<const bf16x8 *>(X + g);
-        }
+        hf.stash(Xs, tid);
         __syncthreads();
+        if (b + (int)gridDim.x < nb) hf.fetch(X, b + gridDim.x, n, tid);
         f32x4 acc[2][2];  // [column][16-point half]
 #pragma unroll
         for (int c = 0; c < 2; c++)
@@ -227,6 +277,11 @@ constexpr int XPL = HX * HY * 48 + 8;        // X^T plane (one ci): z rows of 48
 constexpr size_t DW_LDS = (size_t)(CH * DZS + CH * XPL) * sizeof(bf16);
 
 template <int CIN>
+__device__ __forceinline__ void HaloPrefetch<CIN>::stash_t(bf16 *Xs, int tid) const {
+    stash_t_impl<XPL>(Xs, tid);
+}
+
+template <int CIN>
 __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
                                                  float *__restrict__ part, float *__restrict__ bpart, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -247,29 +302,40 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
     float bacc = 0.0f;
     const int nbz = n / 32, nby = n / 4;
     const int nb = (n / 4) * nby * nbz;
-    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    HaloPrefetch<CIN> hf;
+    constexpr int NPD = 512 * 4 / CONV_T;  // dZ chunks of 8 channels per thread
+    bf16x8 pd[NPD];
+    auto fetch_dz = [&](int b) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * 4, y0 = by * 4, z0 = bz * 32;
-        __syncthreads();
-        for (int i = tid; i < 512 * 4; i += CONV_T) {
+#pragma unroll
+        for (int k = 0; k < NPD; k++) {
+            const int i = tid + k * CONV_T;
             const int q = i & 3, pt = i >> 2;
             const size_t p = ((size_t)(x0 + (pt >> 7)) * n + y0 + ((pt >> 5) & 3)) * n + z0 + (pt & 31);
-            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
-#pragma unroll
-            for (int j = 0; j < 8; j++) Ds[(8 * q + j) * DZS + pt] = v[j];
+            pd[k] = *reinterpret_cast<const bf16x8 *>(dZ + p * CH + 8 * q);
         }
-        constexpr int NQ = CIN / 8;
-        for (int i = tid; i < HX * HY * HZ * NQ; i += CONV_T) {
-            const int q = i % NQ, hp = i / NQ;
-            const int hz = hp % HZ, hy = (hp / HZ) % HY, hx = hp / (HZ * HY);
-            const int gx = wrap(x0 + hx - 1, n), gy = wrap(y0 + hy - 1, n), gz = wrap(z0 + hz - 1, n);
-            const bf16x8 v =
-                *reinterpret_cast<const bf16x8 *>(X + (((size_t)gx * n + gy) * n + gz) * CIN + 8 * q);
-            const int e = (hx * HY + hy) * 48 + 7 + hz;
-#pragma unroll
-            for (int j = 0; j < 8; j++) Xs[(8 * q + j) * XPL + e] = v[j];
-        }
+    };
+    if (blockIdx.x < nb) {
+        fetch_dz(blockIdx.x);
+        hf.fetch(X, blockIdx.x, n, tid);
+    }
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPD; k++) {
+            const int i = tid + k * CONV_T;
+            const int q = i & 3, pt = i >> 2;
+#pragma unroll
+            for (int j = 0; j < 8; j++) Ds[(8 * q + j) * DZS + pt] = pd[k][j];
+        }
+        hf.stash_t(Xs, tid);
+        __syncthreads();
+        // the next brick's loads are in flight during this brick's matrix work
+        if (b + (int)gridDim.x < nb) {
+            fetch_dz(b + gridDim.x);
+            hf.fetch(X, b + gridDim.x, n, tid);
+        }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Ds + bco * DZS + (bseg + 16 * r) * 8);
