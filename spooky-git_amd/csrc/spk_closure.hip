@@ -12,11 +12,13 @@
 //
 // Convolution as a GEMM per 4x4x32 brick of points: D[co][point] +=
 // W[tap][co][ci-chunk] x X[point + tap][ci-chunk], 27 taps x (CIN / 16)
-// K-chunks of 16 channels.  A block (4 waves, one x each, four y columns of 32
-// z points per wave) keeps all 27 taps' weights in LDS for its lifetime and
-// stages each brick's input with a one-point halo; blocks loop over bricks.
-// 16-byte LDS reads are conflict-free through an XOR swizzle of the 8-channel
-// chunks by point index.  The weight gradient contracts over points instead:
+// K-chunks of 16 channels.  A block (8 waves: wave w takes x plane w % 4 and
+// the two y columns 2 (w / 4), 2 (w / 4) + 1 of 32 z points) keeps all 27 taps'
+// weights in LDS for its lifetime and stages each brick's input with a
+// one-point halo; blocks loop over bricks.  16-byte LDS reads are
+// conflict-free: the weights through an XOR swizzle of their 8-channel chunks
+// by output channel, the halo at a padded point pitch.  The weight gradient
+// contracts over points instead:
 // dW[tap][co][ci] = sum_p dZ[p][co] X[p + tap][ci], with both operands staged
 // channel-major so that a lane's 8 K values are 8 consecutive z points; the
 // +-1 z shift of a tap is made with 16-bit funnel shifts of aligned loads.
@@ -44,21 +46,51 @@ constexpr int NBLK = 256;                // persistent blocks (one per CU)
 __host__ __device__ constexpr int swz_shift(int cin) { return cin == 32 ? 2 : 3; }
 __device__ __forceinline__ int swz(int q, int p, int cin) { return q ^ ((p >> swz_shift(cin)) & (cin / 8 - 1)); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erfc(x) for x >= 0 by the Chebyshev fit of Press, Teukolsky, Vetterling and
+// Flannery, Numerical Recipes (2nd ed., sec. 6.2, erfcc): fractional error below
+// 1.2e-7 in exact arithmetic, below 1e-5 evaluated in FP32 up to x = 9; branch
+// free, one reciprocal and one exponential.  The activations it feeds are
+// rounded to BF16 (relative step 2^-8).
+__device__ __forceinline__ float erfc_pos(float x) {
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
+    float p = 0.17087277f;
+    p = fmaf(p, t, -0.82215223f);
+    p = fmaf(p, t, 1.48851587f);
+    p = fmaf(p, t, -1.13520398f);
+    p = fmaf(p, t, 0.27886807f);
+    p = fmaf(p, t, -0.18628806f);
+    p = fmaf(p, t, 0.09678418f);
+    p = fmaf(p, t, 0.37409196f);
+    p = fmaf(p, t, 1.00002368f);
+    p = fmaf(p, t, -1.26551223f);
+    return t * __expf(fmaf(-x, x, p));
+}
+// Phi(z) = erfc(-z / sqrt 2) / 2, the normal distribution function
+__device__ __forceinline__ float norm_cdf(float z) {
+    const float e = 0.5f * erfc_pos(fabsf(z) * 0.70710678118654752f);
+    return z > 0.0f ? 1.0f - e : e;
+}
+// GELU(z) = z Phi(z), GELU'(z) = Phi(z) + z phi(z)
+__device__ __forceinline__ float gelu(float z) { return z * norm_cdf(z); }
 __device__ __forceinline__ float gelu_d(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    return norm_cdf(z) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-constexpr size_t conv_lds(int cin) { return (size_t)(NTAP * CH * cin + HX * HY * HZ * cin) * sizeof(bf16); }
+// Halo points in LDS at a pitch of CIN + 8 channels (80 or 48 bytes): the 16
+// lanes of one 16-byte read cycle, on consecutive points, then fall on disjoint
+// groups of 4 banks (64 banks of 4 bytes), and every tap's address is the lane's
+// base address plus a constant, so the transform needs no per-tap address registers.
+__host__ __device__ constexpr int xpitch(int cin) { return cin + 8; }
+constexpr size_t conv_lds(int cin) { return (size_t)(NTAP * CH * cin + HX * HY * HZ * xpitch(cin)) * sizeof(bf16); }
 
 constexpr int CONV_T = 512;  // 8 waves: two per SIMD, so one wave's LDS reads overlap the other's MFMAs
 
 // One brick's input with its one-point halo (HX x HY x HZ points of CIN
 // channels, channels-last, periodic wrap), held in registers between its global
-// loads (fetch) and its LDS stores (stash: channels-last, chunk-swizzled by halo
-// point; stash_t: channel-major rows of the weight-gradient layout), so that a
+// loads (fetch) and its LDS stores (stash: channels-last at the xpitch pitch;
+// stash_t: channel-major rows of the weight-gradient layout), so that a
 // block loads brick b + 1 while it computes brick b.
 template <int CIN>
 struct HaloPrefetch {
@@ -88,7 +120,7 @@ struct HaloPrefetch {
             const int i = tid + k * CONV_T;
             if (i < NCH) {
                 const int q = i % NQ, hp = i / NQ;
-                *reinterpret_cast<bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN)) = v[k];
+                *reinterpret_cast<bf16x8 *>(Xs + hp * xpitch(CIN) + 8 * q) = v[k];
             }
         }
     }
@@ -121,7 +153,7 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
                                                  float *__restrict__ out, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     bf16 *Ws = reinterpret_cast<bf16 *>(smem);  // [27][32][CIN], chunk-swizzled by co
-    bf16 *Xs = Ws + NTAP * CH * CIN;            // [HX*HY*HZ][CIN], chunk-swizzled by halo point
+    bf16 *Xs = Ws + NTAP * CH * CIN;            // [HX*HY*HZ][xpitch(CIN)]
     constexpr int NQ = CIN / 8;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 31, half = lane >> 5;
@@ -143,8 +175,10 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
         __syncthreads();
         hf.stash(Xs, tid);
         __syncthreads();
-        // the next brick's loads are in flight during this brick's matrix work
+        // the next brick's loads are in flight during this brick's matrix work (the
+        // scheduling barrier keeps them ahead of it)
         if (b + (int)gridDim.x < nb) hf.fetch(X, b + gridDim.x, n, tid);
+        __builtin_amdgcn_sched_barrier(0);
         f32x16 acc[2];
 #pragma unroll
         for (int ys = 0; ys < 2; ys++)
@@ -160,7 +194,7 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
 #pragma unroll
                 for (int ys = 0; ys < 2; ys++) {
                     const int hp = ((xw + 1 + dx) * HY + (yg + ys + 1 + dy)) * HZ + li + 1 + dz;
-                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * CIN + 8 * swz(q, hp, CIN));
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * xpitch(CIN) + 8 * q);
                     acc[ys] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[ys], 0, 0, 0);
                 }
             }
@@ -205,13 +239,13 @@ __global__ __launch_bounds__(CONV_T) void k_conv(const bf16 *__restrict__ X, con
 // one instruction per tap and 16-point tile (the 6 stresses in rows 0..5), half
 // the work of a 32-row tile.  out[c][p] = D + b for c < 6.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr size_t OUT_LDS = (size_t)(NTAP * 16 * CH + HX * HY * HZ * CH) * sizeof(bf16);
+constexpr size_t OUT_LDS = (size_t)(NTAP * 16 * CH + HX * HY * HZ * xpitch(CH)) * sizeof(bf16);
 
 __global__ __launch_bounds__(CONV_T) void k_conv_out(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
                                                      const float *__restrict__ bias, float *__restrict__ out, int n) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     bf16 *Ws = reinterpret_cast<bf16 *>(smem);  // [27][16][32], chunk-swizzled by co
-    bf16 *Xs = Ws + NTAP * 16 * CH;             // [HX*HY*HZ][32], chunk-swizzled by halo point
+    bf16 *Xs = Ws + NTAP * 16 * CH;             // [HX*HY*HZ][xpitch(32)]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int l16 = lane & 15, q = lane >> 4;   // 16x16x32: lane holds row/column l16, K chunk q
     const int xw = w & 3, yg = 2 * (w >> 2);
@@ -234,6 +268,7 @@ __global__ __launch_bounds__(CONV_T) void k_conv_out(const bf16 *__restrict__ X,
         hf.stash(Xs, tid);
         __syncthreads();
         if (b + (int)gridDim.x < nb) hf.fetch(X, b + gridDim.x, n, tid);
+        __builtin_amdgcn_sched_barrier(0);
         f32x4 acc[2][2];  // [column][16-point half]
 #pragma unroll
         for (int c = 0; c < 2; c++)
@@ -250,7 +285,7 @@ __global__ __launch_bounds__(CONV_T) void k_conv_out(const bf16 *__restrict__ X,
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const int hp = ((xw + 1 + dx) * HY + (yg + c + 1 + dy)) * HZ + 16 * h + l16 + 1 + dz;
-                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * CH + 8 * swz(q, hp, CH));
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(Xs + hp * xpitch(CH) + 8 * q);
                     acc[c][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[c][h], 0, 0, 0);
                 }
         }
@@ -336,6 +371,7 @@ __global__ __launch_bounds__(CONV_T) void k_conv_dw(const bf16 *__restrict__ X, 
             fetch_dz(b + gridDim.x);
             hf.fetch(X, b + gridDim.x, n, tid);
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Ds + bco * DZS + (bseg + 16 * r) * 8);
