@@ -644,6 +644,7 @@ __global__ __launch_bounds__(256) void k_unshear(const cpx<double> *__restrict__
 namespace spk {
 bool fused_supported_nx(int nx);
 bool rkl_fused_supported_nx(int nx);
+int launch_pass_y_rkl(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int dir, void *buf, const KGeom &kg);
 int launch_fwdx_rkl2(spk_ctx ctx, int bulk, int nx, const void *Q, void *U, void *Uc0, void *Uc1, void *dU0, void *W,
                      const RklConst &c, int stage, double mu, double nu, double dtmut, double gdt, int inv,
                      double scale, const void *tw, const RklPlan *plan, const TimeState *ts);
@@ -689,6 +690,7 @@ struct spk_solver_s {
     long long *force_idx;
     cpx<double> *force_val;
     InvProg inv_rkl_b, inv_rkl_g, inv_th;
+    InvProg inv_rkl_t;  // fused RKL2 stage 1: i k_x theta, theta
     cpx<double> *rUc0, *rUc1, *rdU0;
     TimeState *ts;
     double *pmax, *part, *res, *spec;
@@ -1165,22 +1167,22 @@ bool rkl_fused(spk_solver s) {
 }
 
 // the part of a fused RKL2 stage before its forward x pass: (stage 1 only) the
-// inverse x pass of i k theta, then the inverse y pass, the flux in the z
-// kernel and the forward y pass of the flux
+// inverse x pass of i k_x theta and theta, then the inverse y pass that makes
+// the three gradient components, the flux in the z kernel and the forward y
+// pass that folds k_y F_y + k_z F_z into one slot
 int rkl2_front_fused(spk_solver s, bool with_inv_x) {
     const Grid3 &g = s->g;
     const size_t fb = (size_t)g.ntc * esize(s->bulk);
-    void *wg = (char *)s->W2 + 3 * fb;  // the i k theta slots of the fused z program
+    void *wg = (char *)s->W2 + 3 * fb;  // the grad theta slots of the fused z program
     if (with_inv_x) {
-        if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_g, 3, wg, 0.0));
-        else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_g, 3, wg, 0.0));
+        if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_t, 2, wg, 0.0));
+        else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_t, 2, wg, 0.0));
     }
-    PassArgs a{wg, wg, 3, g.ntc, g.ntc, 1.0};
-    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+    const KGeom kg = kgeom(s, 0.0);
+    SPK_TRY(launch_pass_y_rkl(s->ctx, g, s->twb, +1, wg, kg));
     if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
     else SPK_TRY(launch_zmid<float>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
-    PassArgs b{s->P2, s->P2, 3, g.ntc, g.ntc, 1.0};
-    return launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, b);
+    return launch_pass_y_rkl(s->ctx, g, s->twb, -1, s->P2, kg);
 }
 
 // forward x pass of the flux + stage update + (inv) the next stage's inverse x pass
@@ -1511,6 +1513,9 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     memset(&s->inv_rkl_b, 0, sizeof(s->inv_rkl_b));
     memset(&s->inv_rkl_g, 0, sizeof(s->inv_rkl_g));
     memset(&s->inv_th, 0, sizeof(s->inv_th));
+    memset(&s->inv_rkl_t, 0, sizeof(s->inv_rkl_t));
+    s->inv_rkl_t.s[0] = InvSlot{s->TH, 0};
+    s->inv_rkl_t.s[1] = InvSlot{s->TH, -1};
     s->inv_th.s[0] = InvSlot{s->TH, -1};
     for (int q = 0; q < 3; q++) s->inv_rkl_b.s[q] = InvSlot{s->BX + q, -1};
     for (int q = 0; q < 3; q++) s->inv_rkl_g.s[q] = InvSlot{s->TH, q};
