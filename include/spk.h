@@ -127,11 +127,15 @@ int spk_main_loop(spk_solver s, int nsteps);
 /* The fused back end of an RK3 stage runs the forward x pass, the spectral
  * epilogue, the RK3 update, the next step's CleanFieldDivergence and the next
  * stage's inverse x pass in one kernel; spk_main_loop can use it where the
- * configuration allows (no stratification, shearing, sts or in-RHS
- * conduction; 32 <= NX <= 1024).  mode 0: never; 1 (default): when the tile
- * autotune run at spk_solver_create found it faster than the unfused kernels
- * on this grid; 2: always where allowed.  All give the same results to rounding. */
+ * configuration allows (no shearing or in-RHS conduction; 32 <= NX <= 1024).
+ * mode 0: never; 1 (default): when the tile autotune run at spk_solver_create
+ * found it faster than the unfused kernels on this grid; 2: always where
+ * allowed.  All give the same results to rounding. */
 int spk_solver_set_fused(spk_solver s, int mode);
+/* Points per thread of the fused kernel's x line: 4 or 8, or 0 for the tile
+ * autotune's choice (the default).  All give the same results to rounding.
+ * No counterpart in the reference (its x passes are cuFFT's). */
+int spk_solver_set_fused_points(spk_solver s, int points);
 /* RKL2 cycles planned on the device (on = 1): the stage count and coefficients
  * of RKLegendre::compute_cycle_RKL2 (rkl.cu:466-585) are computed by a one-thread
  * kernel from the step's dt and dt_sts, and the cycle launches the stage count

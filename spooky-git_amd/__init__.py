@@ -99,6 +99,7 @@ SIGNATURES = {
     "spk_main_loop": [_VP, _I],
     "spk_kernel_timings": [_VP, _I, _VP, _I, _VP],
     "spk_solver_set_fused": [_VP, _I],
+    "spk_solver_set_fused_points": [_VP, _I],
     "spk_solver_pipeline": [_VP, _VP, _VP],
     "spk_energy": [_VP, _I, _VP],
     "spk_enstrophy": [_VP, _I, _VP],
@@ -380,6 +381,10 @@ class Solver:
     def set_fused(self, mode):
         """0: unfused kernels; 1: autotuned choice (default); 2: fused kernel wherever allowed."""
         check(lib().spk_solver_set_fused(self.h, int(mode)))
+
+    def set_fused_points(self, points):
+        """Points per thread of the fused kernel's x line: 4, 8, or 0 for the autotuned tile."""
+        check(lib().spk_solver_set_fused_points(self.h, int(points)))
 
     def pipeline(self):
         """(fused back end used by main_loop, points per thread of its x line)."""

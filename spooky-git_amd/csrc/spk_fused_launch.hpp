@@ -20,9 +20,8 @@ static void launch_ly_mode(int mode, dim3 grid, int bs, hipStream_t st, const cp
     }
 }
 
-// Boussinesq layouts: stratified runs take the velocity + temperature group,
-// instantiated with 4 points per thread only (with 8 its 4 accumulated
-// components spill registers)
+// Boussinesq layouts: stratified runs couple the velocity and temperature
+// groups in one block (k_fwdx_epilogue's STRAT instantiation)
 template <class T, int N, int LY, int EF, bool BAT>
 static int launch_bouss(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T> *P, cpx<double> *U, cpx<T> *S,
                         cpx<T> *W, const FusedConst &fc, const cpx<T> *tw, const TimeState *ts, const Batch &bt) {
@@ -30,12 +29,8 @@ static int launch_bouss(int mode, dim3 grid, int bs, hipStream_t st, const cpx<T
         launch_ly_mode<T, N, LY, EF, BAT>(mode, grid, bs, st, P, U, S, W, fc, tw, ts, bt);
         return SPK_OK;
     }
-    if constexpr (EF == 4) {
-        launch_ly_mode<T, N, LY, EF, BAT, true>(mode, grid, bs, st, P, U, S, W, fc, tw, ts, bt);
-        return SPK_OK;
-    }
-    set_error("fused epilogue: stratified runs use 4 points per thread");
-    return SPK_ERR_ARG;
+    launch_ly_mode<T, N, LY, EF, BAT, true>(mode, grid, bs, st, P, U, S, W, fc, tw, ts, bt);
+    return SPK_OK;
 }
 
 template <class T, int N, int EF, bool BAT>

@@ -530,7 +530,7 @@ __global__ __launch_bounds__(256) void k_epilogue(const cpx<T> *__restrict__ P, 
 // k.T, k x emf or k.(u theta) is accumulated in registers, so the transformed
 // products never go back to HBM.  The groups run in different blocks and update
 // their fields in place, so no group reads another group's state (stratified
-// runs take velocity and temperature as one group).  Not used with shearing
+// runs take the temperature and velocity groups in the same blocks).  Not used with shearing
 // (sheared k between stages, remap) or in-RHS conduction (its grad theta slots
 // come from k_inv_x).
 // ------------------------------------------------------------------------
@@ -571,15 +571,19 @@ struct FusedConst {
 #define SPK_WAIT_LGKMCNT0() __builtin_amdgcn_s_waitcnt(0xC07F)
 
 // One field group of k_fwdx_epilogue: GRP 0 velocity (5 products T_ij), 1
-// magnetic (3 products emf), 2 temperature (3 products u theta), 3 velocity and
-// temperature together (8 products), for stratification: BoussinesqStrat
-// (cuda_kernels.cu:310-325) couples u_s and theta, and fields updated in place
-// may only be read by the group that updates them.  Every lane runs the same
+// magnetic (3 products emf), 2 temperature (3 products u theta).  Fields
+// updated in place may only be read by the group that updates them, and
+// BoussinesqStrat (cuda_kernels.cu:310-325) couples u_s and theta.  STRAT
+// (stratified runs): the temperature group and then the velocity group run in
+// one block; the temperature group reads u_s's current state (not yet updated)
+// and leaves theta's current state at its points in W's first velocity slot,
+// which the velocity group reads back before its inverse x pass writes that
+// slot.  Every lane runs the same
 // straight-line code: a lane past the last column works on the last column and
 // stores the same values as that column's own lane, so no memory operation sits
 // behind a branch.  Element offsets inside a field fit 32 bits and are added to
 // wave-uniform field bases.
-template <class T, int N, int LY, int MODE, int EF, int GRP>
+template <class T, int N, int LY, int MODE, int EF, int GRP, bool STRAT = false>
 __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ R,
                                          cpx<T> *__restrict__ W, const FusedConst &fc,
                                          const cpx<T> *__restrict__ tw, cpx<T> *twl, cpx<T> *ldsc, cpx<T> *pbuf,
@@ -587,12 +591,9 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     using L = Layout<LY>;
     constexpr int E = fx_e(N, EF);
     constexpr int TPL = N / E;
-    constexpr int NPROD = GRP == 0 ? 5 : (GRP == 3 ? 8 : 3);
-    constexpr int NCOMP = GRP == 2 ? 1 : (GRP == 3 ? 4 : 3);
-    constexpr int F0 = (GRP == 0 || GRP == 3) ? L::VX : (GRP == 1 ? L::BX : L::TH);
-    constexpr bool VEL = GRP == 0 || GRP == 3;
-    // field of component q (GRP 3: u_x, u_y, u_z, theta)
-    auto fld = [](int q) { return (GRP == 3 && q == 3) ? L::TH : F0 + q; };
+    constexpr int NPROD = GRP == 0 ? 5 : 3;
+    constexpr int NCOMP = GRP == 2 ? 1 : 3;
+    constexpr int F0 = GRP == 0 ? L::VX : (GRP == 1 ? L::BX : L::TH);
     // FP64 bulk: product lines reach LDS by direct global->LDS loads
     // (global_load_lds_dwordx4), which take no registers.  Lane l of wave w puts
     // slot i at pbuf[(w E + i) 64 + l] and reads back only its own slots, so only
@@ -605,7 +606,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     const unsigned ucol = (unsigned)col;
     const int j = (int)(col / c.kg.nzh);
     const int kk = (int)(col - (long long)j * c.kg.nzh);
-    const int p0 = VEL ? c.oT : (GRP == 1 ? c.oE : c.oQ);
+    const int p0 = GRP == 0 ? c.oT : (GRP == 1 ? c.oE : c.oQ);
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     // ky, kz are constant along the x line
@@ -623,8 +624,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 
     cpx<T> vn[DMA ? 1 : E];
     auto fetch = [&](int q) {
-        // GRP 3: T_ij, then u theta
-        const cpx<T> *src = P + (long long)((GRP == 3 && q >= 5) ? c.oQ + q - 5 : p0 + q) * N0;
+        const cpx<T> *src = P + (long long)(p0 + q) * N0;
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const cpx<T> *g = src + (ucol + (unsigned)(t + i * TPL) * ncols);
@@ -660,26 +660,20 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         LineFFT<T, N, E, -1>::run(v, ldsc, t, twl);
         // coefficient of product q in each accumulated component, as a select on
         // kx (per point) and ky, kz (per line)
-        const bool q0 = q == 0, q1 = q == 1, q2 = q == 2, q3 = q == 3, q4 = q == 4, q5 = q == 5, q6 = q == 6;
+        const bool q0 = q == 0, q1 = q == 1, q2 = q == 2, q3 = q == 3, q4 = q == 4;
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const int n = t + i * TPL;
             const double kx = c.kg.kfx * (double)(n < c.kg.nx / 2 ? n : n - c.kg.nx);  // no shear here
             const cpx<double> a = cconv<double>(v[i]);
-            if constexpr (VEL) {
-                // NonLinHydroAdv: k_j T_ij with T_zz = -T_xx - T_yy (zero coefficients for
-                // GRP 3's u theta products)
+            if constexpr (GRP == 0) {
+                // NonLinHydroAdv: k_j T_ij with T_zz = -T_xx - T_yy
                 const double c0 = q0 ? kx : (q1 ? ky : (q2 ? kz : 0.0));
                 const double c1 = q1 ? kx : (q3 ? ky : (q4 ? kz : 0.0));
                 const double c2 = (q0 || q3) ? -kz : (q2 ? kx : (q4 ? ky : 0.0));
                 acc[0][i] = cpx<double>{acc[0][i].x + c0 * a.x, acc[0][i].y + c0 * a.y};
                 acc[1][i] = cpx<double>{acc[1][i].x + c1 * a.x, acc[1][i].y + c1 * a.y};
                 acc[2][i] = cpx<double>{acc[2][i].x + c2 * a.x, acc[2][i].y + c2 * a.y};
-                if constexpr (GRP == 3) {
-                    // NonLinBoussinesqAdv: k.(u theta) from products 5..7
-                    const double c3 = q5 ? kx : (q6 ? ky : (q >= 7 ? kz : 0.0));
-                    acc[3][i] = cpx<double>{acc[3][i].x + c3 * a.x, acc[3][i].y + c3 * a.y};
-                }
             } else if constexpr (GRP == 1) {
                 // MagneticShear: k x emf
                 const double c0 = q1 ? -kz : (q2 ? ky : 0.0);
@@ -703,15 +697,23 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
     const double gam = c.gam, ga = c.gam * dt;
     const double cu = fc.cu, cs = fc.cs;
     constexpr int PD = 1;
-    cpx<double> su[E][NCOMP];
-    cpx<T> sr[E][NCOMP];
+    // STRAT: the temperature group also loads u_s (component 1), the velocity
+    // group theta's current state parked by the temperature group
+    constexpr bool SX = STRAT && GRP == 2;
+    constexpr bool SV = STRAT && GRP == 0;
+    constexpr int NLD = NCOMP + (SX ? 1 : 0);
+    cpx<double> su[E][NLD];
+    cpx<T> sr[E][NLD];
+    cpx<T> pk[SV ? E : 1];
     auto load_state = [&](int i) {
         const unsigned pi = ucol + (unsigned)(t + i * TPL) * ncols;
 #pragma unroll
-        for (int q = 0; q < NCOMP; q++) {
-            su[i][q] = (U + fld(q) * N0)[pi];
-            if constexpr (MODE != 1) sr[i][q] = (R + fld(q) * N0)[pi];
+        for (int q = 0; q < NLD; q++) {
+            const int f = q < NCOMP ? F0 + q : L::VX + c.sd;
+            su[i][q] = (U + f * N0)[pi];
+            if constexpr (MODE != 1) sr[i][q] = (R + f * N0)[pi];
         }
+        if constexpr (SV) pk[i] = (W + L::VX * N0)[pi];
     };
 #pragma unroll
     for (int i = 0; i < PD && i < E; i++) load_state(i);
@@ -724,9 +726,9 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         kvec_at(c.kg, n, j, kk, kx, kyi, kzi, m);
         const double k2 = kx * kx + ky * ky + kz * kz;
         // the stage's current state (diffusion) and the base of its update
-        cpx<double> uc[NCOMP], base[NCOMP];
+        cpx<double> uc[NLD], base[NLD];
 #pragma unroll
-        for (int q = 0; q < NCOMP; q++) {
+        for (int q = 0; q < NLD; q++) {
             if constexpr (MODE == 1) {
                 uc[q] = su[i][q];
                 base[q] = su[i][q];
@@ -737,17 +739,15 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             }
         }
         cpx<double> d[NCOMP];
-        if constexpr (VEL) {
+        if constexpr (GRP == 0) {
 #pragma unroll
             for (int q = 0; q < 3; q++) d[q] = mul_minus_im(m, acc[q][i]);
-            if constexpr (GRP == 3) {
-                // BoussinesqStrat: d u_s -= theta, d theta += N2 u_s (current state)
+            if constexpr (SV) {
+                // BoussinesqStrat: d u_s -= theta (current state)
+                const cpx<double> th = cconv<double>(pk[i]);
 #pragma unroll
                 for (int q = 0; q < 3; q++)
-                    if (q == c.sd) d[q] = cpx<double>{d[q].x - uc[3].x, d[q].y - uc[3].y};
-                const cpx<double> us = c.sd == 0 ? uc[0] : (c.sd == 1 ? uc[1] : uc[2]);
-                d[3] = cadd(mul_minus_im(m, acc[3][i]), cscale(us, c.N2));
-                d[3] = diffuse(c.diff_mask, L::TH, d[3], uc[3], -c.nu_th * k2);
+                    if (q == c.sd) d[q] = cpx<double>{d[q].x - th.x, d[q].y - th.y};
             }
             const double f = -c.nu * k2;
 #pragma unroll
@@ -768,6 +768,12 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             for (int q = 0; q < 3; q++) d[q] = diffuse(c.diff_mask, L::BX + q, d[q], uc[q], f);
         } else {
             d[0] = mul_minus_im(m, acc[0][i]);
+            if constexpr (SX) {
+                // BoussinesqStrat: d theta += N2 u_s (current state); theta's current
+                // state parked for the velocity group
+                d[0] = cadd(d[0], cscale(uc[1], c.N2));
+                (W + L::VX * N0)[pi] = cconv<T>(uc[0]);
+            }
             d[0] = diffuse(c.diff_mask, L::TH, d[0], uc[0], -c.nu_th * k2);
         }
         cpx<double> un[NCOMP];
@@ -776,14 +782,14 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
             if constexpr (MODE < 3) {
                 // the increment in the bulk precision, and the update from that rounded value
                 const cpx<T> rq = cconv<T>(cscale(d[q], dt));
-                (R + fld(q) * N0)[pi] = rq;
+                (R + (F0 + q) * N0)[pi] = rq;
                 const cpx<double> r = cconv<double>(rq);
                 un[q] = cpx<double>{base[q].x + gam * r.x, base[q].y + gam * r.y};
             } else {
                 un[q] = cpx<double>{base[q].x + ga * d[q].x, base[q].y + ga * d[q].y};
             }
         }
-        if constexpr (GRP != 2) {
+        if constexpr (GRP < 2) {
             if (fc.project) {
                 // CleanDivergence of the new u / B (the next step's CleanFieldDivergence)
                 const cpx<double> sd3 = kdot3(kx, un[0], ky, un[1], kz, un[2]);
@@ -799,7 +805,7 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
         }
 #pragma unroll
         for (int q = 0; q < NCOMP; q++) {
-            if constexpr (MODE != 1) (U + fld(q) * N0)[pi] = un[q];  // stage 1 keeps U0
+            if constexpr (MODE != 1) (U + (F0 + q) * N0)[pi] = un[q];  // stage 1 keeps U0
             acc[q][i] = un[q];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -812,15 +818,15 @@ __device__ __forceinline__ void fx_group(const cpx<T> *P, cpx<double> *__restric
 #pragma unroll
             for (int i = 0; i < E; i++) v[i] = cconv<T>(cscale(acc[q][i], fc.scale));
             LineFFT<T, N, E, +1>::run(v, ldsc, t, twl);
-            cpx<T> *dst = W + (long long)fld(q) * N0;
+            cpx<T> *dst = W + (long long)(F0 + q) * N0;
 #pragma unroll
             for (int i = 0; i < E; i++) dst[ucol + (unsigned)(t + i * TPL) * ncols] = v[i];
         }
     }
 }
 
-// STRAT (Boussinesq layouts with stratification): velocity and temperature in
-// one group, grid y = 1 + MHD
+// STRAT (Boussinesq layouts with stratification): the temperature group and
+// then the velocity group in the first grid row's blocks, grid y = 1 + MHD
 template <class T, int N, int LY, int MODE, int EF, bool BAT = false, bool STRAT = false>
 __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilogue(
     const cpx<T> *P, cpx<double> *__restrict__ U, cpx<T> *__restrict__ S, cpx<T> *__restrict__ W,
@@ -855,10 +861,18 @@ __global__ __launch_bounds__(fx_tcols(N, EF) * fx_tpl(N, EF)) void k_fwdx_epilog
     // temperature (boussinesq without STRAT)
     const int g = blockIdx.y;
     if (g == 0) {
-        if constexpr (L::BOUSS && STRAT)
-            fx_group<T, N, LY, MODE, EF, 3>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
-        else
+        if constexpr (L::BOUSS && STRAT) {
+            fx_group<T, N, LY, MODE, EF, 2, true>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
+            // the velocity group's per-point values are computed anew, not kept
+            // live across the temperature group
+            int t2 = t;
+            long long col2 = col;
+            asm volatile("" : "+v"(t2));
+            asm volatile("" : "+v"(col2));
+            fx_group<T, N, LY, MODE, EF, 0, true>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t2, col2);
+        } else {
             fx_group<T, N, LY, MODE, EF, 0>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
+        }
     } else if (L::MHD && g == 1) {
         fx_group<T, N, LY, MODE, EF, 1>(P, U, S, W, fc, tw, twl, ldsc, pbuf, ts, tid, NT, t, col);
     } else if constexpr (L::BOUSS && !STRAT) {

@@ -705,6 +705,7 @@ struct spk_solver_s {
     int fused_mode;      // caller's switch for k_fwdx_epilogue: 0 never, 1 autotuned, 2 always
     bool fused_faster;   // tile autotune: the fused back end beat the unfused one on this grid
     int fused_ef;        // points per thread of its x line
+    int fused_ef_auto;   // the tile autotune's choice of fused_ef
 };
 
 namespace {
@@ -1285,6 +1286,7 @@ int hydro_mhd_advance(spk_solver s) {
 // products and a zero step keep every buffer at zero.
 int tune_fused(spk_solver s) {
     s->fused_ef = 4;
+    s->fused_ef_auto = 4;
     s->fused_faster = false;
     if (!fused_allowed(s)) return SPK_OK;
     const Grid3 &g = s->g;
@@ -1313,11 +1315,7 @@ int tune_fused(spk_solver s) {
     };
     float best = 1e30f, t_unf = 0.f, t_cl = 0.f;
     int rc = SPK_OK;
-    // stratified runs: 4 points per thread only (k_fwdx_epilogue's velocity +
-    // temperature group)
-    const bool strat = s->p.boussinesq && s->p.stratification;
     for (int ef : {4, 8}) {
-        if (strat && ef != 4) continue;
         float ms = 0.f;
         rc = time_it([&] {
             return launch_fwdx_epilogue(s->ctx, s->bulk, g.nx, s->ly, 2, ef, s->P, s->U, s->S, s->W, fc, tw, s->ts,
@@ -1344,6 +1342,7 @@ int tune_fused(spk_solver s) {
     (void)hipEventDestroy(e1);
     SPK_TRY(rc);
     s->fused_faster = best < t_unf + t_cl / 3.0f;
+    s->fused_ef_auto = s->fused_ef;
     return SPK_OK;
 }
 
@@ -2360,6 +2359,7 @@ int solver_follow_back_end(spk_solver s, spk_solver lead) {
     s->fused_mode = lead->fused_mode;
     s->fused_faster = lead->fused_faster;
     s->fused_ef = lead->fused_ef;
+    s->fused_ef_auto = lead->fused_ef_auto;
     return SPK_OK;
 }
 }  // namespace spk
@@ -2397,6 +2397,14 @@ extern "C" int spk_solver_set_fused(spk_solver s, int mode) {
     SPK_CHECK_SOLVER(s);
     SPK_REQUIRE(mode >= 0 && mode <= 2, SPK_ERR_ARG, "spk_solver_set_fused: mode %d", mode);
     s->fused_mode = mode;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_set_fused_points(spk_solver s, int points) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(points == 0 || points == 4 || points == 8, SPK_ERR_ARG,
+                "spk_solver_set_fused_points: %d (4, 8, or 0 for the autotuned tile)", points);
+    s->fused_ef = points == 0 ? s->fused_ef_auto : points;
     return SPK_OK;
 }
 

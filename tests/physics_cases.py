@@ -103,7 +103,7 @@ def relerr(a, b):
 
 
 # cases on which spk_main_loop takes the fused path (32 <= NX <= 1024, no shearing or in-RHS
-# conduction): the stratified ones run velocity and temperature as one group of the fused kernel,
+# conduction): the stratified ones run the temperature and velocity groups in the same blocks,
 # the super-time-stepped ones interleave RKL2 cycles (theta alone: its inverse x pass is made again
 # after an even step's cycle; velocity: the next step starts from a fresh projection and transform)
 FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d", "mhd_strat_x32", "hd_strat_x32",

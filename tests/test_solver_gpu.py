@@ -190,14 +190,18 @@ def test_taylor_green_64_100_steps(ctx):
     assert sol.time()["step"] == 100
 
 
+@pytest.mark.parametrize("points", [4, 8])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("name", FUSED_CASES)
-def test_fused_path_matches_unfused_and_oracle(ctx, name, prec):
-    """The fused forward-x + epilogue + projection + next-inverse-x kernel gives the
-    unfused pipeline's results (to rounding) and the oracle's, over several steps
-    (the W buffer handed from one step to the next is exercised from step 2 on)."""
+def test_fused_path_matches_unfused_and_oracle(ctx, name, prec, points):
+    """The fused forward-x + epilogue + projection + next-inverse-x kernel, at 4 and
+    at 8 points per thread, gives the unfused pipeline's results (to rounding) and
+    the oracle's, over several steps (the W buffer handed from one step to the next
+    is exercised from step 2 on)."""
     sim, sol, u0 = make_pair(ctx, name, prec)
     sol.set_fused(2)
+    sol.set_fused_points(points)
+    assert sol.pipeline() == (True, points)
     ref2 = spk.Solver(ctx, prec, **CASES[name]["params"])
     ref2.set_fused(0)
     ref2.set_fields(u0)
@@ -212,6 +216,7 @@ def test_fused_path_matches_unfused_and_oracle(ctx, name, prec):
     # two calls in a row (fresh W at the start of each call) equal one longer call
     sol2 = spk.Solver(ctx, prec, **CASES[name]["params"])
     sol2.set_fused(2)
+    sol2.set_fused_points(points)
     sol2.set_fields(u0)
     sol2.main_loop(2)
     sol2.main_loop(3)
