@@ -161,3 +161,50 @@ def test_fft_512_roundtrip(ctx):
     out = d.download(np.float64, (n, n, 2 * (n // 2 + 1)))[..., :n]
     assert relerr(out, ref) < 1e-12
     plan.destroy()
+
+
+# MTI 256^3 (the bench's MTI workload: spooky-git_amd.initial.mti_params and
+# mti_white_noise, seed 7): Boussinesq MHD with stratification along z and
+# anisotropic conduction of theta advanced by RKL2 (14 stages per cycle).  Two
+# main-loop steps cover both orders of HydroMHDAdvance (hydro_mhd_advance.cu:
+# 18-75): step 0 runs RK3 and then the RKL2 cycle, step 1 the cycle and then
+# RK3.  Fused mode 2 runs the stratified fused RK3 kernel (temperature and
+# velocity groups in one block) and the fused RKL2 stage; mode 0 the unfused
+# kernels.  Each field group (u, B, theta) against the FP64 oracle to 1e-10
+# relative L2: u starts at zero and is driven by buoyancy alone, so a group's
+# own norm is the scale.
+MTI_STEPS = 2
+
+
+@pytest.fixture(scope="module")
+def mti_case():
+    p = ini.mti_params(N)
+    u0 = ini.mti_white_noise((N, N, N), seed=7)
+    print("\n# oracle: 256^3 MTI, %d steps" % MTI_STEPS, flush=True)
+    sim = orc.Sim(**p)
+    sim.set_fields(u0)
+    sim.main_loop(MTI_STEPS)
+    res = dict(params=p, u0=u0, fields=sim.fields.copy(), time=sim.time())
+    del sim
+    return res
+
+
+@pytest.mark.parametrize("fused", [2, 0])
+def test_mti256_steps_vs_oracle(ctx, mti_case, fused):
+    print("# gpu: 256^3 MTI, fused mode %d" % fused, flush=True)
+    sol = spk.Solver(ctx, "f64", **mti_case["params"])
+    sol.set_fused(fused)
+    assert sol.pipeline()[0] == (fused == 2)
+    sol.set_fields(mti_case["u0"])
+    sol.main_loop(MTI_STEPS)
+    got, ref = sol.get_fields(), mti_case["fields"]
+    v = sol.vars
+    groups = {"u": slice(v["VX"], v["VX"] + 3), "B": slice(v["BX"], v["BX"] + 3), "theta": slice(v["TH"], v["TH"] + 1)}
+    for name, sl in groups.items():
+        err = relerr(got[sl], ref[sl])
+        assert err < 1e-10, (fused, name, err)
+    assert np.linalg.norm(np.ravel(ref[groups["u"]])) > 0.0
+    t = sol.time()
+    assert t["step"] == MTI_STEPS
+    assert abs(t["t"] - mti_case["time"]["t"]) <= 1e-13 * mti_case["time"]["t"]
+    sol.destroy()
