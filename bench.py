@@ -20,6 +20,8 @@ Beside the timed line (not part of `value`):
   taylor_green_64            64^3 HD Taylor-Green, 100 steps, GPU and oracle;
   mti_rkl2                   256^3 Boussinesq MTI with anisotropic conduction,
                              FP64 RKL2 on theta: steps/s and stages per cycle;
+                             the same with FP32 bulk (state and RKL2 recurrence
+                             FP64): steps/s and difference from FP64;
   ensemble                   48 MHD boxes at 64^3 + 16 at 128^3 + 32 Boussinesq-MHD
                              boxes at 512x512x2 with RKL2 conduction, FP32 bulk,
                              one batch per shape (spk_ensemble): steps of every box
@@ -316,25 +318,39 @@ def taylor_green_leg(spk, ini, torch, ctx, cpu):
 def mti_leg(spk, ini, torch, ctx, n, steps, warmup, cpu):
     """256^3 Boussinesq MTI with anisotropic conduction, FP64, RKL2 on theta
     (see spooky-git_amd.initial.mti_params): steps/s with HIP events, and the
-    RKL2 stage count of the run's dt / dt_sts; the FP64 oracle times one step."""
+    RKL2 stage count of the run's dt / dt_sts; the same run with FP32 bulk
+    (transforms and products in FP32, the RK3 state and the RKL2 recurrence in
+    FP64): steps/s and each field group's relative L2 difference from the FP64
+    state after the same steps; the FP64 oracle times one step."""
+    import numpy as np
     p = ini.mti_params(n)
     state = ini.mti_white_noise((n, n, n), seed=7)
-    sol = spk.Solver(ctx, "f64", **p)
-    sol.set_fields(state)
-    sol.main_loop(warmup)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    sol.main_loop(steps)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    tm = sol.time()
-    sol.destroy()
+    out = {}
+    for prec in ("f64", "f32"):
+        sol = spk.Solver(ctx, prec, **p)
+        sol.set_fields(state)
+        sol.main_loop(warmup)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sol.main_loop(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        out[prec] = dict(ms=e0.elapsed_time(e1) / steps, time=sol.time(), fields=sol.get_fields(), vars=sol.vars)
+        sol.destroy()
+    ms, tm, v = out["f64"]["ms"], out["f64"]["time"], out["f64"]["vars"]
+    a, b = out["f32"]["fields"], out["f64"]["fields"]
+    groups = {"u": slice(v["VX"], v["VX"] + 3), "B": slice(v["BX"], v["BX"] + 3), "theta": slice(v["TH"], v["TH"] + 1)}
+    diff = {g: float(np.linalg.norm((a[sl] - b[sl]).ravel()) / max(np.linalg.norm(b[sl].ravel()), 1e-300))
+            for g, sl in groups.items()}
     res = dict(grid=[n, n, n], steps=steps, ms_per_step=ms, steps_per_s=1000.0 / ms, dt=tm["dt"],
                dt_sts=tm["dt_sts"], rkl2_stages=rkl2_stages(tm["dt"], tm["dt_sts"]),
+               fp32_bulk=dict(ms_per_step=out["f32"]["ms"], steps_per_s=1000.0 / out["f32"]["ms"],
+                              ratio_vs_f64=ms / out["f32"]["ms"], steps_compared=warmup + steps,
+                              rel_l2_vs_f64=diff),
                params={k: p[k] for k in ("nu", "nu_m", "nu_th", "reynolds_ani", "N2", "OmegaT2", "lx")},
                ic="u=0, B=(1e-4,0,0), theta white noise uniform in [-1e-4,1e-4], seed 7")
+    del out, a, b
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import pyoracle as orc
