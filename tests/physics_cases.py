@@ -68,6 +68,14 @@ CASES = {
                                            strat_direction=1, N2=0.5, nu=0.02, nu_th=0.01, cfl=0.5,
                                            supertimestepping=1, sts_mask=0b111, safety_sts=10, cfl_par=1.0,
                                            lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
+    "mhd_x1024": dict(params=dict(nx=1024, ny=8, nz=8, incompressible=1, mhd=1, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI,
+                                  nu=0.01, nu_m=0.02, cfl=0.5, **BIG_T), fields="ub"),
+    "mhd_bouss_rkl2_x512": dict(params=dict(nx=512, ny=8, nz=8, incompressible=1, mhd=1, boussinesq=1,
+                                            stratification=1, strat_direction=2, anisotropic_diffusion=1, N2=0.1,
+                                            OmegaT2=1.0, reynolds_ani=30.0, nu=0.01, nu_m=0.01, nu_th=0.01,
+                                            cfl=0.5, supertimestepping=1, sts_mask=1 << 6, safety_sts=20,
+                                            cfl_par=1.0, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T),
+                                fields="ubt", b0=1.0),
     "hd_bouss_x32": dict(params=dict(nx=32, ny=16, nz=8, incompressible=1, boussinesq=1, nu=0.01, nu_th=0.01,
                                      cfl=0.5, lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
     "mhd_bouss_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1, nu=0.01,
@@ -107,4 +115,4 @@ def relerr(a, b):
 # the super-time-stepped ones interleave RKL2 cycles (theta alone: its inverse x pass is made again
 # after an even step's cycle; velocity: the next step starts from a fresh projection and transform)
 FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d", "mhd_strat_x32", "hd_strat_x32",
-               "mhd_bouss_rkl2_x32", "hd_strat_sts_u_x32"]
+               "mhd_bouss_rkl2_x32", "hd_strat_sts_u_x32", "mhd_x1024", "mhd_bouss_rkl2_x512"]
