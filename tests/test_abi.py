@@ -53,3 +53,24 @@ def test_snapshot_library_exports_spk_io_h():
                      "spkio_last_snapshot"}
     L = snapshot.lib()
     assert all(hasattr(L, n) for n in names)
+
+
+def test_null_handles_and_bad_arguments_are_refused():
+    """Entry points validate their arguments before any device work: null handles
+    and out-of-range choices return SPK_ERR_ARG with a message naming the call
+    (no GPU is needed for these paths)."""
+    L = spk.lib()
+    L.spk_last_error.restype = ctypes.c_char_p
+    err_arg = -1
+    calls = [
+        ("spk_solver_set_fused", lambda: L.spk_solver_set_fused(None, 1)),
+        ("spk_solver_set_fused_points", lambda: L.spk_solver_set_fused_points(None, 8)),
+        ("spk_main_loop", lambda: L.spk_main_loop(None, 1)),
+        ("spk_solver_create", lambda: L.spk_solver_create(None, None, 0, None)),
+        ("spk_ctx_create", lambda: L.spk_ctx_create(0, None, None)),
+        ("spk_energy_noalloc", lambda: L.spk_energy_noalloc(None, 8, 8, 8, None, None)),
+    ]
+    for name, call in calls:
+        assert call() == err_arg, name
+        msg = L.spk_last_error().decode()
+        assert name in msg, (name, msg)
