@@ -136,6 +136,16 @@ int spk_solver_set_fused(spk_solver s, int mode);
  * autotune's choice (the default).  All give the same results to rounding.
  * No counterpart in the reference (its x passes are cuFFT's). */
 int spk_solver_set_fused_points(spk_solver s, int points);
+/* Quasi-2D grids (NZ = 2, NY = 64..512): the inverse y pass, the real-space
+ * products and the forward y pass of an RHS evaluation run as one kernel per x
+ * plane, and so does the flux of a fused RKL2 stage.  on = 1 (default) uses
+ * them where the grid and RHS program allow, 0 never.  Both give the same
+ * results to rounding.  No counterpart in the reference (its y and z
+ * transforms are cuFFT's). */
+int spk_solver_set_plane(spk_solver s, int on);
+/* Whether spk_main_loop runs the plane kernel in its RK3 stages (*rk3) and in
+ * its RKL2 stages (*rkl2). */
+int spk_solver_plane(spk_solver s, int *rk3, int *rkl2);
 /* RKL2 cycles planned on the device (on = 1): the stage count and coefficients
  * of RKLegendre::compute_cycle_RKL2 (rkl.cu:466-585) are computed by a one-thread
  * kernel from the step's dt and dt_sts, and the cycle launches the stage count

@@ -100,6 +100,8 @@ SIGNATURES = {
     "spk_kernel_timings": [_VP, _I, _VP, _I, _VP],
     "spk_solver_set_fused": [_VP, _I],
     "spk_solver_set_fused_points": [_VP, _I],
+    "spk_solver_set_plane": [_VP, _I],
+    "spk_solver_plane": [_VP, _VP, _VP],
     "spk_solver_pipeline": [_VP, _VP, _VP],
     "spk_energy": [_VP, _I, _VP],
     "spk_enstrophy": [_VP, _I, _VP],
@@ -385,6 +387,16 @@ class Solver:
     def set_fused_points(self, points):
         """Points per thread of the fused kernel's x line: 4, 8, or 0 for the autotuned tile."""
         check(lib().spk_solver_set_fused_points(self.h, int(points)))
+
+    def set_plane(self, on):
+        """Quasi-2D grids: the plane kernels (y passes and products in one kernel) on (default) or off."""
+        check(lib().spk_solver_set_plane(self.h, 1 if on else 0))
+
+    def plane(self):
+        """(plane kernel in the RK3 stages, plane kernel in the RKL2 stages) of main_loop."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().spk_solver_plane(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), bool(b.value)
 
     def pipeline(self):
         """(fused back end used by main_loop, points per thread of its x line)."""

@@ -650,6 +650,12 @@ int launch_fwdx_rkl2(spk_ctx ctx, int bulk, int nx, const void *Q, void *U, void
                      double scale, const void *tw, const RklPlan *plan, const TimeState *ts);
 int launch_fwdx_epilogue(spk_ctx ctx, int bulk, int nx, int ly, int mode, int ef, const void *P, void *U, void *S,
                          void *W, const FusedConst &fc, const void *tw, const TimeState *ts, int ngroups);
+bool plane_supported(const Grid3 &g, int prog);
+bool plane_rkl_supported(const Grid3 &g);
+int launch_plane(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, int prog, const void *W, void *P,
+                 const ZConst &zc, double *pmax);
+int launch_plane_rkl(spk_ctx ctx, const Grid3 &g, const Twiddles &tw, const void *G, const void *Breal, void *Q,
+                     const KGeom &kg, const ZConst &zc);
 }  // namespace spk
 
 // ==========================================================================
@@ -706,6 +712,7 @@ struct spk_solver_s {
     bool fused_faster;   // tile autotune: the fused back end beat the unfused one on this grid
     int fused_ef;        // points per thread of its x line
     int fused_ef_auto;   // the tile autotune's choice of fused_ef
+    bool plane_on;       // caller's switch for the quasi-2D plane kernels (k_plane, k_plane_rkl)
 };
 
 namespace {
@@ -852,20 +859,39 @@ static const double xiRK[2] = {-17.0 / 60.0, -5.0 / 12.0};
 
 // ---- pipeline pieces ----
 
-// inverse transforms of the RHS inputs and the fused z kernel: products in P
-int rhs_front(spk_solver s, double kxshift, bool with_max) {
-    if (s->nin == 0) return SPK_OK;
+ZConst zconst(spk_solver s) { return ZConst{1.0 / s->p.reynolds_ani, s->p.OmegaT2, s->p.strat_direction}; }
+
+// On a quasi-2D grid the inverse y pass, the z kernel and the forward y pass of
+// the products are one kernel (k_plane): P then leaves the front already
+// transformed along y, and the back end starts at its x pass.
+bool plane_front(spk_solver s) { return s->plane_on && s->nin > 0 && plane_supported(s->g, s->zprog); }
+bool plane_rkl(spk_solver s) { return s->plane_on && plane_rkl_supported(s->g); }
+
+// products of the RHS inputs in W (inverse x pass done): inverse y pass + z
+// kernel, or the plane kernel (which also makes the forward y pass)
+int rhs_mid(spk_solver s, bool with_max) {
     const Grid3 &g = s->g;
-    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv, s->nin, s->W, kxshift));
-    else SPK_TRY(launch_inv_x<float>(s, s->inv, s->nin, s->W, kxshift));
+    double *pm = with_max ? s->pmax : nullptr;
+    if (plane_front(s)) {
+        SPK_TRY(launch_plane(s->ctx, g, s->twb, s->zprog, s->W, s->P, zconst(s), pm));
+        s->zmid_blocks = g.nx;
+        return SPK_OK;
+    }
     PassArgs a{s->W, s->W, s->nin, g.ntc, g.ntc, 1.0};
     SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
-    double *pm = with_max ? s->pmax : nullptr;
     unsigned nb = 0;
     if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, s->zprog, s->W, s->P, pm, &nb));
     else SPK_TRY(launch_zmid<float>(s, s->zprog, s->W, s->P, pm, &nb));
     s->zmid_blocks = (int)nb;
     return SPK_OK;
+}
+
+// inverse transforms of the RHS inputs and the fused z kernel: products in P
+int rhs_front(spk_solver s, double kxshift, bool with_max) {
+    if (s->nin == 0) return SPK_OK;
+    if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv, s->nin, s->W, kxshift));
+    else SPK_TRY(launch_inv_x<float>(s, s->inv, s->nin, s->W, kxshift));
+    return rhs_mid(s, with_max);
 }
 
 EpiConst make_epi(spk_solver s, double kxshift, double gam, double xi) {
@@ -892,7 +918,7 @@ int rhs_back(spk_solver s, double kxshift, int mode, double gam, double xi) {
     const Grid3 &g = s->g;
     if (s->nout > 0) {
         PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
-        SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+        if (!plane_front(s)) SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
         SPK_TRY(launch_pass_x(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
     }
     const EpiConst c = make_epi(s, kxshift, gam, xi);
@@ -924,7 +950,7 @@ int fused_groups(spk_solver s) {
 int rhs_back_fused(spk_solver s, int mode, double gam, double xi, int project, int inv) {
     const Grid3 &g = s->g;
     PassArgs a{s->P, s->P, s->nout, g.ntc, g.ntc, 1.0};
-    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
+    if (!plane_front(s)) SPK_TRY(launch_pass_y(s->ctx, g, s->twb, -1, s->bulk, s->bulk, a));
     FusedConst fc;
     memset(&fc, 0, sizeof(fc));
     fc.e = make_epi(s, 0.0, gam, xi);
@@ -941,17 +967,7 @@ int rhs_back_fused(spk_solver s, int mode, double gam, double xi, int project, i
 }
 
 // inverse y pass + fused z kernel from a W that already holds the inverse x pass
-int rhs_front_from_w(spk_solver s, bool with_max) {
-    const Grid3 &g = s->g;
-    PassArgs a{s->W, s->W, s->nin, g.ntc, g.ntc, 1.0};
-    SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
-    double *pm = with_max ? s->pmax : nullptr;
-    unsigned nb = 0;
-    if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, s->zprog, s->W, s->P, pm, &nb));
-    else SPK_TRY(launch_zmid<float>(s, s->zprog, s->W, s->P, pm, &nb));
-    s->zmid_blocks = (int)nb;
-    return SPK_OK;
-}
+int rhs_front_from_w(spk_solver s, bool with_max) { return rhs_mid(s, with_max); }
 
 int dt_kernel(spk_solver s) {
     DtConst c;
@@ -1180,6 +1196,7 @@ int rkl2_front_fused(spk_solver s, bool with_inv_x) {
         else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_t, 2, wg, 0.0));
     }
     const KGeom kg = kgeom(s, 0.0);
+    if (plane_rkl(s)) return launch_plane_rkl(s->ctx, g, s->twb, wg, s->Breal, s->P2, kg, zconst(s));
     SPK_TRY(launch_pass_y_rkl(s->ctx, g, s->twb, +1, wg, kg));
     if (s->bulk == SPK_F64) SPK_TRY(launch_zmid<double>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
     else SPK_TRY(launch_zmid<float>(s, ZP_B | ZP_G | ZP_BR, s->W2, s->P2, nullptr, nullptr));
@@ -1575,6 +1592,7 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     s->loop_started = false;
     s->products_valid = false;
     s->fused_mode = 1;
+    s->plane_on = true;
     s->zmid_blocks = 0;
     return tune_fused(s);
 }
@@ -2352,7 +2370,7 @@ bool solver_same_shape(spk_solver a, spk_solver b) {
                       pa.t_initial == pb.t_initial && pa.t_final == pb.t_final && pa.toutput_flow == pb.toutput_flow;
     return ints && dbls && a->bulk == b->bulk && a->rkl_device == b->rkl_device && a->force_n == 0 &&
            b->force_n == 0 && !a->p.shearing && (a->W2 != nullptr) == (b->W2 != nullptr) &&
-           (a->Breal != nullptr) == (b->Breal != nullptr);
+           (a->Breal != nullptr) == (b->Breal != nullptr) && a->plane_on == b->plane_on;
 }
 
 int solver_follow_back_end(spk_solver s, spk_solver lead) {
@@ -2360,6 +2378,7 @@ int solver_follow_back_end(spk_solver s, spk_solver lead) {
     s->fused_faster = lead->fused_faster;
     s->fused_ef = lead->fused_ef;
     s->fused_ef_auto = lead->fused_ef_auto;
+    s->plane_on = lead->plane_on;
     return SPK_OK;
 }
 }  // namespace spk
@@ -2405,6 +2424,21 @@ extern "C" int spk_solver_set_fused_points(spk_solver s, int points) {
     SPK_REQUIRE(points == 0 || points == 4 || points == 8, SPK_ERR_ARG,
                 "spk_solver_set_fused_points: %d (4, 8, or 0 for the autotuned tile)", points);
     s->fused_ef = points == 0 ? s->fused_ef_auto : points;
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_set_plane(spk_solver s, int on) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(on == 0 || on == 1, SPK_ERR_ARG, "spk_solver_set_plane: %d (0 or 1)", on);
+    s->plane_on = on != 0;
+    s->products_valid = false;  // P's layout after the front depends on the switch
+    return SPK_OK;
+}
+
+extern "C" int spk_solver_plane(spk_solver s, int *rk3, int *rkl2) {
+    SPK_CHECK_SOLVER(s);
+    if (rk3) *rk3 = plane_front(s) ? 1 : 0;
+    if (rkl2) *rkl2 = plane_rkl(s) && s->rkl_aniso && rkl_fused(s) ? 1 : 0;
     return SPK_OK;
 }
 

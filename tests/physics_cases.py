@@ -81,6 +81,21 @@ CASES = {
     "mhd_bouss_x32": dict(params=dict(nx=32, ny=8, nz=16, incompressible=1, mhd=1, boussinesq=1, nu=0.01,
                                       nu_m=0.01, nu_th=0.02, cfl=0.5, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T),
                           fields="ubt"),
+    # quasi-2D grids on which the plane kernels run (NZ = 2, NY = 64..512)
+    "q2d_hd_y64": dict(params=dict(nx=32, ny=64, nz=2, incompressible=1, nu=0.01, cfl=0.5, lx=TWO_PI, ly=TWO_PI,
+                                   lz=TWO_PI, **BIG_T), fields="u"),
+    "q2d_bouss_y64": dict(params=dict(nx=64, ny=64, nz=2, incompressible=1, boussinesq=1, nu=0.01, nu_th=0.02,
+                                      cfl=0.5, lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="ut"),
+    "q2d_mhd_y256": dict(params=dict(nx=32, ny=256, nz=2, incompressible=1, mhd=1, nu=2e-3, nu_m=3e-3, cfl=0.5,
+                                     lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T), fields="ub"),
+    "q2d_mhd_bouss_y128": dict(params=dict(nx=32, ny=128, nz=2, incompressible=1, mhd=1, boussinesq=1, nu=0.01,
+                                           nu_m=0.01, nu_th=0.02, cfl=0.5, lx=TWO_PI, ly=TWO_PI, lz=TWO_PI,
+                                           **BIG_T), fields="ubt"),
+    "q2d_hbi_y512": dict(params=dict(nx=32, ny=512, nz=2, incompressible=1, mhd=1, boussinesq=1, stratification=1,
+                                     strat_direction=1, anisotropic_diffusion=1, N2=0.2, OmegaT2=-1.0,
+                                     reynolds_ani=20.0, nu=2e-3, nu_m=2e-3, nu_th=1e-3, cfl=0.5,
+                                     supertimestepping=1, sts_mask=1 << 6, safety_sts=20, cfl_par=1.0,
+                                     lx=1.0, ly=4.0, lz=1.0, **BIG_T), fields="ubt", b0=1.0),
     "heat": dict(params=dict(nx=64, ny=2, nz=2, heat_equation=1, nu_th=1e-3, antialiasing=0, cfl=1.5,
                              lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="t"),
 }
@@ -116,3 +131,7 @@ def relerr(a, b):
 # after an even step's cycle; velocity: the next step starts from a fresh projection and transform)
 FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d", "mhd_strat_x32", "hd_strat_x32",
                "mhd_bouss_rkl2_x32", "hd_strat_sts_u_x32", "mhd_x1024", "mhd_bouss_rkl2_x512"]
+
+# quasi-2D cases of the plane kernels: (RK3 stages, RKL2 stages) expected on them
+PLANE_CASES = {"q2d_hd_y64": (True, False), "q2d_bouss_y64": (True, False), "q2d_mhd_y256": (True, False),
+               "q2d_mhd_bouss_y128": (True, False), "q2d_hbi_y512": (True, True)}

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import pyoracle as orc
-from physics_cases import CASES, FUSED_CASES, ini, initial_state, relerr
+from physics_cases import CASES, FUSED_CASES, PLANE_CASES, ini, initial_state, relerr
 from spkmod import spk
 
 pytestmark = pytest.mark.gpu
@@ -223,6 +223,33 @@ def test_fused_path_matches_unfused_and_oracle(ctx, name, prec, points):
     assert relerr(sol2.get_fields(), a) < (1e-13 if prec == "f64" else 1e-6)
     t = sol.kernel_timings(2)
     assert t["fwdx_epilogue"] is not None and t["fwdx_epilogue"] > 0
+
+
+@pytest.mark.parametrize("fused", [2, 0])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("name", sorted(PLANE_CASES))
+def test_plane_kernels_match_three_kernel_path(ctx, name, prec, fused):
+    """On quasi-2D grids the plane kernel (inverse y pass, z-row products and
+    forward y pass of the products in one kernel per x plane; for the RKL2 stages
+    the anisotropic flux) gives the state of the separate y-pass and z kernels to
+    rounding, and the oracle's, over several steps of both back ends."""
+    sim, sol, u0 = make_pair(ctx, name, prec)
+    sol.set_fused(fused)
+    ref = spk.Solver(ctx, prec, **CASES[name]["params"])
+    ref.set_fused(fused)
+    ref.set_plane(False)
+    ref.set_fields(u0)
+    rk3, rkl2 = PLANE_CASES[name]
+    assert sol.plane() == (rk3, rkl2 and fused == 2)
+    assert ref.plane() == (False, False)
+    nsteps = 6
+    sim.main_loop(nsteps)
+    sol.main_loop(nsteps)
+    ref.main_loop(nsteps)
+    a, b = sol.get_fields(), ref.get_fields()
+    assert relerr(a, b) < (1e-13 if prec == "f64" else 1e-6), relerr(a, b)
+    assert relerr(a, sim.fields) < TOL_STEPS[prec], relerr(a, sim.fields)
+    assert sol.time()["t"] == pytest.approx(sim.time()["t"], rel=1e-12 if prec == "f64" else 1e-6)
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
