@@ -67,6 +67,8 @@ def parse():
                     help="steps of the FP32-vs-FP64 ensemble accuracy check (0: skip)")
     ap.add_argument("--ensemble-split", type=int, default=-1,
                     help="batches per shape group (0: box by box; -1: autotuned)")
+    ap.add_argument("--ensemble-group", default="all", choices=["all", "mhd64^3", "mhd128^3", "hbi512x512x2"],
+                    help="run one shape group of the ensemble alone (--mode ensemble)")
     ap.add_argument("--closure-spinup", type=float, default=2.0)
     ap.add_argument("--closure-iters", type=int, default=3000)
     ap.add_argument("--closure-dns", type=int, default=512)
@@ -407,7 +409,7 @@ def ensemble_bytes(p, bulk_bytes):
     return 2 * transforms * sb + 7 * nf * s64, 6 * 2 * sb + 6 * s64 + 3 * sb
 
 
-def ensemble_leg(spk, ini, ctx, steps, acc_steps=10, split=-1):
+def ensemble_leg(spk, ini, ctx, steps, acc_steps=10, split=-1, group="all"):
     """The ensemble in FP32 bulk.  Boxes of one shape run as one batch (one
     launch per kernel for all of them); the batch split is autotuned, then
     `steps` steps of every box are timed (host clock around run + sync) with the
@@ -418,6 +420,9 @@ def ensemble_leg(spk, ini, ctx, steps, acc_steps=10, split=-1):
     each kind is checked against the FP64 oracle."""
     import numpy as np
     boxes, kinds = ensemble_boxes(ini)
+    if group != "all":
+        keep = [i for i, k in enumerate(kinds) if k == group]
+        boxes, kinds = [boxes[i] for i in keep], [kinds[i] for i in keep]
     t0 = time.perf_counter()
     ens = spk.Ensemble(boxes, "f32")
     states = []
@@ -506,7 +511,7 @@ def ensemble_leg(spk, ini, ctx, steps, acc_steps=10, split=-1):
                 achieved_gbs=round(step_b / (graph_ms * 1e-3) / 1e9, 1),
                 frac_of_hbm_peak=round(step_b / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 eager_batched_ms_per_step=round(eager_ms, 3), box_by_box_16_lanes_ms_per_step=None if box_ms is None else round(box_ms, 3),
-                rkl2_stages_hbi=[min(stages), max(stages)], setup_s=round(setup_s, 1),
+                rkl2_stages_hbi=[min(stages), max(stages)] if stages else None, setup_s=round(setup_s, 1),
                 accuracy_f32_vs_f64=dict(steps=acc_steps, per_kind=acc), f64_vs_oracle=oracle,
                 params="nu, eta log-uniform [1e-4, 1e-2]; chi_par uniform [1e-3, 1e-1] (hbi); seeds 5000+i")
 
@@ -764,7 +769,8 @@ def main():
         print(json.dumps(mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu)))
         return
     if a.mode == "ensemble":
-        print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split)))
+        print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split,
+                                      a.ensemble_group)))
         return
     if a.mode == "closure-kernels":
         print(json.dumps(closure_kernels(spk, ctx, torch)))
