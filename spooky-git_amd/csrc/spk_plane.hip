@@ -181,8 +181,8 @@ __global__ __launch_bounds__(64 * plane_nw<PROG>()) void k_plane(const cpx<T> *_
 }
 
 // One RKL2 stage's flux, three waves: inverse y of A, of i k_y B and of B
-// (times i k_z after it); products; forward y of F_x, and of F_y and F_z
-// together (H).  G: slots A, B; Br: B's real-space rows (3 slots); Q: Y, H.
+// (times i k_z after it); products; forward y of F_x, F_y and F_z, the last two
+// combined into H.  G: slots A, B; Br: B's real-space rows (3 slots); Q: Y, H.
 template <class T, int NY, bool BAT>
 __global__ __launch_bounds__(192) void k_plane_rkl(const cpx<T> *__restrict__ G, const cpx<T> *__restrict__ Br,
                                                   cpx<T> *__restrict__ Q, long long fs, KGeom kg,
@@ -262,30 +262,30 @@ __global__ __launch_bounds__(192) void k_plane_rkl(const cpx<T> *__restrict__ G,
     }
     __syncthreads();
 
-    // forward y: wave 0 Y = FFT_y F_x; wave 1 H = k_y FFT_y F_y + k_z FFT_y F_z
-    // (F_z's region is free: wave 2 has no forward transform)
-    if (w < 2) {
+    // forward y: wave 0 Y = FFT_y F_x; waves 1 and 2 transform F_y and F_z, wave 2
+    // leaves k_z FFT_y F_z in its region and wave 1 adds k_y FFT_y F_y to it: H
 #pragma unroll
-        for (int i = 0; i < E; i++) v[i] = R[PadSkew::at(t + i * PL_TPL)];
-        cpx<T> u[E];
-        if (w == 1) {
-            const cpx<T> *Rz = reg + (4 + c) * LL;
+    for (int i = 0; i < E; i++) v[i] = R[PadSkew::at(t + i * PL_TPL)];
+    lsync<true>();
+    PlaneFFT<T, NY, -1>::run(v, R, t, twl);
+    if (w == 0) {
+        cpx<T> *dst = Q + plane + c;
 #pragma unroll
-            for (int i = 0; i < E; i++) u[i] = Rz[PadSkew::at(t + i * PL_TPL)];
-        }
+        for (int i = 0; i < E; i++) dst[2 * (t + i * PL_TPL)] = v[i];
+    } else if (w == 2) {
         lsync<true>();
-        PlaneFFT<T, NY, -1>::run(v, R, t, twl);
-        cpx<T> *dst = Q + (w == 0 ? 0 : fs) + plane + c;
-        if (w == 0) {
 #pragma unroll
-            for (int i = 0; i < E; i++) dst[2 * (t + i * PL_TPL)] = v[i];
-        } else {
-            PlaneFFT<T, NY, -1>::run(u, reg + (4 + c) * LL, t, twl);
+        for (int i = 0; i < E; i++) R[PadSkew::at(t + i * PL_TPL)] = cpx<T>{kz * v[i].x, kz * v[i].y};
+    }
+    __syncthreads();
+    if (w == 1) {
+        const cpx<T> *Rz = reg + (4 + c) * LL;
+        cpx<T> *dst = Q + fs + plane + c;
 #pragma unroll
-            for (int i = 0; i < E; i++) {
-                const T ky = ky_at(t + i * PL_TPL);
-                dst[2 * (t + i * PL_TPL)] = cpx<T>{ky * v[i].x + kz * u[i].x, ky * v[i].y + kz * u[i].y};
-            }
+        for (int i = 0; i < E; i++) {
+            const T ky = ky_at(t + i * PL_TPL);
+            const cpx<T> z = Rz[PadSkew::at(t + i * PL_TPL)];
+            dst[2 * (t + i * PL_TPL)] = cpx<T>{ky * v[i].x + z.x, ky * v[i].y + z.y};
         }
     }
 }
