@@ -22,6 +22,9 @@ Beside the timed line (not part of `value`):
                              FP64 RKL2 on theta: steps/s and stages per cycle;
                              the same with FP32 bulk (state and RKL2 recurrence
                              FP64): steps/s and difference from FP64;
+  hbi2d_512                  problems/hbi2d at 512x512x2 (Boussinesq MHD, RKL2
+                             conduction), one box per precision with and without
+                             the quasi-2D plane kernels; FP64 against the oracle;
   ensemble                   48 MHD boxes at 64^3 + 16 at 128^3 + 32 Boussinesq-MHD
                              boxes at 512x512x2 with RKL2 conduction, FP32 bulk,
                              one batch per shape (spk_ensemble): steps of every box
@@ -79,11 +82,12 @@ def parse():
     ap.add_argument("--no-closure", dest="closure", action="store_false",
                     help="skip the closure leg of the default run (a shortened DNS: spin-up to t = 0.5, "
                          "6 samples 0.05 apart, 1500 training iterations per seed, 0.25 time units compared)")
-    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "ensemble", "closure", "closure-kernels"],
+    ap.add_argument("--mode", choices=["full", "passes", "kernels", "mti", "hbi2d", "ensemble", "closure",
+                                       "closure-kernels"],
                     default="full",
                     help="passes: only the FFT pass kernels and a copy kernel; kernels: each pipeline "
                          "kernel of the 256^3 FP64 step twice, plus the copy kernel (both for counter runs); "
-                         "mti: only the MTI leg (--mti-steps steps)")
+                         "mti: only the MTI leg (--mti-steps steps); hbi2d: only the problems/hbi2d leg")
     return ap.parse_args()
 
 
@@ -361,6 +365,55 @@ def mti_leg(spk, ini, torch, ctx, n, steps, warmup, cpu):
         t0 = time.perf_counter()
         sim.main_loop(1)
         res.update(cpu_s_per_step=time.perf_counter() - t0, cpu_cores=orc.lib().orc_num_threads())
+    return res
+
+
+def hbi2d_leg(spk, ini, torch, ctx, steps, warmup, cpu):
+    """problems/hbi2d at its own size (512 x 512 x 2, Boussinesq MHD with
+    stratification along y and anisotropic conduction advanced by RKL2 on theta;
+    spooky-git_amd.initial.hbi_params / hbi_noise): steps/s of one box in each
+    precision with the quasi-2D plane kernels and without them (the separate y
+    passes and z kernel), and the FP64 state against the oracle after 2 steps."""
+    import numpy as np
+    n = 512
+    p = ini.hbi_params(n)
+    state = ini.hbi_noise((n, n, 2), seed=11)
+    res = dict(grid=[n, n, 2], steps=steps)
+    for prec in ("f64", "f32"):
+        for plane in (True, False):
+            sol = spk.Solver(ctx, prec, **p)
+            sol.set_plane(plane)
+            sol.set_fields(state)
+            sol.main_loop(warmup)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            sol.main_loop(steps)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            key = prec + ("_plane" if plane else "_three_kernels")
+            res[key] = dict(ms_per_step=round(ms, 4), steps_per_s=round(1000.0 / ms, 1), plane=sol.plane())
+            if plane and prec == "f64":
+                tm = sol.time()
+                res["rkl2_stages"] = rkl2_stages(tm["dt"], tm["dt_sts"])
+            sol.destroy()
+    res["plane_speedup"] = {pr: round(res[pr + "_three_kernels"]["ms_per_step"] / res[pr + "_plane"]["ms_per_step"], 3)
+                            for pr in ("f64", "f32")}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle as orc
+        sol = spk.Solver(ctx, "f64", **p)
+        sol.set_fields(state)
+        sol.main_loop(2)
+        sim = orc.Sim(**p)
+        sim.set_fields(state)
+        t0 = time.perf_counter()
+        sim.main_loop(2)
+        res["cpu_s_per_step"] = round((time.perf_counter() - t0) / 2, 3)
+        r = sim.fields
+        res["f64_rel_l2_vs_oracle_2_steps"] = float(np.linalg.norm(sol.get_fields() - r) / np.linalg.norm(r))
+        sol.destroy()
     return res
 
 
@@ -768,6 +821,9 @@ def main():
     if a.mode == "mti":
         print(json.dumps(mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu)))
         return
+    if a.mode == "hbi2d":
+        print(json.dumps(hbi2d_leg(spk, ini, torch, ctx, 40, 2, not a.no_cpu)))
+        return
     if a.mode == "ensemble":
         print(json.dumps(ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split,
                                       a.ensemble_group)))
@@ -838,6 +894,9 @@ def main():
     mti = mti_leg(spk, ini, torch, ctx, n, a.mti_steps, 2, not a.no_cpu) if a.extra else None
     if mti:
         print("# mti %d^3: %s" % (n, json.dumps(mti)), file=sys.stderr, flush=True)
+    q2d = hbi2d_leg(spk, ini, torch, ctx, 40, 2, not a.no_cpu) if a.extra else None
+    if q2d:
+        print("# hbi2d 512x512x2: %s" % json.dumps(q2d), file=sys.stderr, flush=True)
     ens = ensemble_leg(spk, ini, ctx, a.ensemble_steps, a.ensemble_acc_steps, a.ensemble_split) if a.extra and a.ensemble_steps > 0 else None
     closure = None
     if a.extra and a.closure:
@@ -883,6 +942,7 @@ def main():
                        "accuracy_vs_f64": acc}
                       if "f32" in res and "f64" in res else None),
         "taylor_green_64": tg,
+        "hbi2d_512": q2d,
         "mti_rkl2": mti,
         "ensemble": ens,
         "closure": closure,
