@@ -713,6 +713,12 @@ struct spk_solver_s {
     int fused_ef;        // points per thread of its x line
     int fused_ef_auto;   // the tile autotune's choice of fused_ef
     bool plane_on;       // caller's switch for the quasi-2D plane kernels (k_plane, k_plane_rkl)
+    int wslot_b;         // W slot of B_x in the RHS program (-1: none)
+    // where the next RKL2 cycle takes B's real-space rows from: 0 the state (inverse
+    // x, y and z passes); 1 W's B slots, which hold B's inverse x and y passes;
+    // 2 W's B slots, which hold its inverse x pass (set by the fused step around
+    // its cycles, where W holds the current state)
+    int breal_from_w;
 };
 
 namespace {
@@ -1148,6 +1154,18 @@ int rkl2_front(spk_solver s) {
 // inverse x, inverse y and z C2R of the three B fields into Breal.
 int rkl2_b_real(spk_solver s) {
     const Grid3 &g = s->g;
+    if (s->breal_from_w != 0 && s->wslot_b >= 0) {
+        // B's inverse x pass is already in W (the step's RHS program): no pass over
+        // the state, and W itself is left as it is
+        const void *wb = (const char *)s->W + (size_t)s->wslot_b * (size_t)g.ntc * esize(s->bulk);
+        if (s->breal_from_w == 2) {
+            PassArgs a{wb, s->W2, 3, g.ntc, g.ntc, 1.0};
+            SPK_TRY(launch_pass_y(s->ctx, g, s->twb, +1, s->bulk, s->bulk, a));
+            wb = s->W2;
+        }
+        PassArgs z{wb, s->Breal, 3, g.ntc, g.ntc, 1.0};
+        return launch_pass_z_c2r(s->ctx, g, s->twb, s->bulk, s->bulk, z);
+    }
     const double ks = s->tremap * s->p.shear;
     if (s->bulk == SPK_F64) SPK_TRY(launch_inv_x<double>(s, s->inv_rkl_b, 3, s->W2, ks));
     else SPK_TRY(launch_inv_x<float>(s, s->inv_rkl_b, 3, s->W2, ks));
@@ -1383,7 +1401,15 @@ int step_fused(spk_solver s, bool w_valid) {
     // compute_dt: the stage-1 transforms carry the CFL maxima
     SPK_TRY(rhs_front_from_w(s, true));
     SPK_TRY(dt_kernel(s));
-    if (sts && (s->step % 2) == 1) SPK_TRY(cycle());
+    // W holds the inverse x pass of the current state (and, after the separate
+    // inverse y pass, its y pass too): the cycle takes B's real rows from there
+    auto cycle_from_w = [&](int mode) {
+        s->breal_from_w = mode;
+        const int rc = cycle();
+        s->breal_from_w = 0;
+        return rc;
+    };
+    if (sts && (s->step % 2) == 1) SPK_TRY(cycle_from_w(plane_front(s) ? 2 : 1));
     SPK_TRY(rhs_back_fused(s, 1, gammaRK[0], xiRK[0], 0, 1));
     SPK_TRY(rhs_front_from_w(s, false));
     SPK_TRY(rhs_back_fused(s, 2, gammaRK[1], xiRK[1], 0, 1));
@@ -1392,7 +1418,7 @@ int step_fused(spk_solver s, bool w_valid) {
     SPK_TRY(clock(s, 1, 0, 0, 0, 0.0));
     s->fused_w_ok = true;
     if (sts && (s->step % 2) == 0) {
-        SPK_TRY(cycle());
+        SPK_TRY(cycle_from_w(2));  // W: stage 3's inverse x pass of the new state
         if (s->nsts == 1 && s->sts_index[0] == s->TH) {
             // theta's W slot is its field index (the RHS program's slot order)
             const Grid3 &g = s->g;
@@ -1516,6 +1542,9 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
         }
     }
     s->nin = ns;
+    s->wslot_b = -1;
+    for (int q = 0; q < ns; q++)
+        if (s->inv.s[q].src == s->BX && s->inv.s[q].deriv < 0 && s->wslot_b < 0) s->wslot_b = q;
     s->oT = 0;
     s->oE = inc ? 5 : 0;
     s->oQ = s->oE + (inc && p.mhd ? 3 : 0);
@@ -1593,6 +1622,7 @@ int create(spk_ctx ctx, const spk_params *pin, int bulk, spk_solver s) {
     s->products_valid = false;
     s->fused_mode = 1;
     s->plane_on = true;
+    s->breal_from_w = 0;
     s->zmid_blocks = 0;
     return tune_fused(s);
 }
