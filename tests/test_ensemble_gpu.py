@@ -63,15 +63,16 @@ def test_device_rkl_plan_matches_host(ctx):
 
 def boxes():
     """Uneven boxes: MHD 16^3 and 32x16x16 with different coefficients, and two
-    32x32x2 Boussinesq-MHD boxes with anisotropic conduction and RKL2 on theta."""
+    64x64x2 Boussinesq-MHD boxes with anisotropic conduction and RKL2 on theta
+    (quasi-2D: their RK3 and RKL2 stages run the plane kernels)."""
     T = 2 * np.pi
     big = dict(t_final=1e9, toutput_flow=1e9)
     mhd = dict(incompressible=1, mhd=1, lx=T, ly=T, lz=T, cfl=0.5, **big)
     out = [dict(mhd, nx=16, ny=16, nz=16, nu=3e-3, nu_m=1e-2),
            dict(mhd, nx=32, ny=16, nz=16, nu=1e-2, nu_m=2e-3),
            dict(mhd, nx=16, ny=16, nz=16, nu=5e-3, nu_m=5e-3),
-           ini.hbi_params(32, nu=2e-3, eta=1e-3, chi=0.05),
-           ini.hbi_params(32, nu=1e-3, eta=3e-3, chi=0.02)]
+           ini.hbi_params(64, nu=2e-3, eta=1e-3, chi=0.05),
+           ini.hbi_params(64, nu=1e-3, eta=3e-3, chi=0.02)]
     return out
 
 
@@ -144,7 +145,7 @@ def test_ensemble_fp32_and_recapture(ctx):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_batched_launches_match_box_by_box(ctx, prec):
-    """Shape groups of boxes() (two 16^3 MHD boxes, one 32x16x16, two 32x32x2
+    """Shape groups of boxes() (two 16^3 MHD boxes, one 32x16x16, two 64x64x2
     HBI boxes with RKL2): one batch per group, two batches per group, and box by
     box give bit-identical states and clocks after 6 steps (graph on, then 2
     more steps without it)."""
