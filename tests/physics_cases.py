@@ -96,6 +96,8 @@ CASES = {
                                      reynolds_ani=20.0, nu=2e-3, nu_m=2e-3, nu_th=1e-3, cfl=0.5,
                                      supertimestepping=1, sts_mask=1 << 6, safety_sts=20, cfl_par=1.0,
                                      lx=1.0, ly=4.0, lz=1.0, **BIG_T), fields="ubt", b0=1.0),
+    "q2d_mhd_y1024": dict(params=dict(nx=32, ny=1024, nz=2, incompressible=1, mhd=1, nu=2e-3, nu_m=3e-3, cfl=0.5,
+                                      lx=TWO_PI, ly=TWO_PI, lz=TWO_PI, **BIG_T), fields="ub"),
     "heat": dict(params=dict(nx=64, ny=2, nz=2, heat_equation=1, nu_th=1e-3, antialiasing=0, cfl=1.5,
                              lx=1.0, ly=1.0, lz=1.0, **BIG_T), fields="t"),
 }
@@ -134,4 +136,5 @@ FUSED_CASES = ["mhd_x32", "hd_x64", "hd_bouss_x32", "mhd_bouss_x32", "quasi2d", 
 
 # quasi-2D cases of the plane kernels: (RK3 stages, RKL2 stages) expected on them
 PLANE_CASES = {"q2d_hd_y64": (True, False), "q2d_bouss_y64": (True, False), "q2d_mhd_y256": (True, False),
-               "q2d_mhd_bouss_y128": (True, False), "q2d_hbi_y512": (True, True)}
+               "q2d_mhd_bouss_y128": (True, False), "q2d_hbi_y512": (True, True),
+               "q2d_mhd_y1024": (False, False)}  # NY = 1024: the separate y passes and z kernel

@@ -65,6 +65,8 @@ def test_null_handles_and_bad_arguments_are_refused():
     calls = [
         ("spk_solver_set_fused", lambda: L.spk_solver_set_fused(None, 1)),
         ("spk_solver_set_fused_points", lambda: L.spk_solver_set_fused_points(None, 8)),
+        ("spk_solver_set_plane", lambda: L.spk_solver_set_plane(None, 1)),
+        ("spk_solver_plane", lambda: L.spk_solver_plane(None, None, None)),
         ("spk_main_loop", lambda: L.spk_main_loop(None, 1)),
         ("spk_solver_create", lambda: L.spk_solver_create(None, None, 0, None)),
         ("spk_ctx_create", lambda: L.spk_ctx_create(0, None, None)),
