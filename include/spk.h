@@ -123,6 +123,17 @@ int spk_hydro_mhd_advance(spk_solver s);
  * (supervisor.cu:169-190): CleanFieldDivergence + HydroMHDAdvance, with the
  * snapshot clock of CheckSnapshot (check_output.cu:62-94) kept on the device */
 int spk_main_loop(spk_solver s, int nsteps);
+/* As spk_main_loop, continuing the previous spk_main_loop or spk_main_loop_resume
+ * call on s: the caller has not changed the state since (only spk_solver_time and
+ * other reads in between, no writes through spk_solver_buffers' pointers).  On
+ * the fused back end the first step then starts from the inverse x pass of the
+ * state that the last step left (whose stage 3 also made the projection),
+ * instead of a fresh CleanFieldDivergence and inverse x pass: a caller that
+ * steps one at a time between outputs, as Supervisor::executeMainLoop
+ * (supervisor.cu:169-213) does, gets the throughput of one long call.  Any
+ * other entry point taking s in between makes the next call start fresh.  No
+ * counterpart in the reference. */
+int spk_main_loop_resume(spk_solver s, int nsteps);
 
 /* The fused back end of an RK3 stage runs the forward x pass, the spectral
  * epilogue, the RK3 update, the next step's CleanFieldDivergence and the next

@@ -101,6 +101,7 @@ SIGNATURES = {
     "spk_solver_set_fused": [_VP, _I],
     "spk_solver_set_fused_points": [_VP, _I],
     "spk_solver_set_plane": [_VP, _I],
+    "spk_main_loop_resume": [_VP, _I],
     "spk_solver_plane": [_VP, _VP, _VP],
     "spk_solver_pipeline": [_VP, _VP, _VP],
     "spk_energy": [_VP, _I, _VP],
@@ -324,6 +325,10 @@ class Solver:
     # state transfer
     def set_fields(self, c):
         c = np.ascontiguousarray(c, dtype=np.complex128).reshape((self.nf,) + self.shape_c)
+        # the buffer query also tells the library that the state is about to change
+        # (the next main_loop_resume then starts from a fresh inverse x pass)
+        f, d, r = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().spk_solver_buffers(self.h, ctypes.byref(f), ctypes.byref(d), ctypes.byref(r)))
         upload_ptr(self.ctx, self.d_fields, c)
 
     def get_fields(self):
@@ -375,8 +380,11 @@ class Solver:
     def advance(self):
         check(lib().spk_hydro_mhd_advance(self.h))
 
-    def main_loop(self, nsteps):
-        check(lib().spk_main_loop(self.h, nsteps))
+    def main_loop(self, nsteps, resume=False):
+        """nsteps iterations of the main loop.  resume: continue the previous call,
+        the state unchanged since (spk_main_loop_resume): the first step reuses the
+        inverse x pass the last fused step left, where there is one."""
+        check((lib().spk_main_loop_resume if resume else lib().spk_main_loop)(self.h, nsteps))
 
     KERNEL_CLASSES = ["inv_x", "inv_y", "zmid", "fwd_y", "fwd_x", "epilogue", "clean_div", "dt", "fwdx_epilogue"]
 

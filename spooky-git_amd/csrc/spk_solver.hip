@@ -877,6 +877,7 @@ bool plane_rkl(spk_solver s) { return s->plane_on && plane_rkl_supported(s->g); 
 // kernel, or the plane kernel (which also makes the forward y pass)
 int rhs_mid(spk_solver s, bool with_max) {
     const Grid3 &g = s->g;
+    s->fused_w_ok = false;  // W no longer holds the last fused step's inverse x pass as it left it
     double *pm = with_max ? s->pmax : nullptr;
     if (plane_front(s)) {
         SPK_TRY(launch_plane(s->ctx, g, s->twb, s->zprog, s->W, s->P, zconst(s), pm));
@@ -1646,7 +1647,17 @@ unsigned red_grid(long long n) {
 // C ABI
 // ==========================================================================
 
-#define SPK_CHECK_SOLVER(s) SPK_REQUIRE((s) != nullptr, SPK_ERR_ARG, "%s: null solver", __func__)
+// Every entry point taking a solver checks it and, since it may change the
+// state or the transform buffers, drops the continuation of the last fused
+// step (the inverse x pass it left in W); only spk_main_loop_resume keeps it.
+// entry points that read the state and write neither it nor W (times, queries,
+// diagnostics into their own scratch and the real-space buffer)
+#define SPK_CHECK_SOLVER_RO(s) SPK_REQUIRE((s) != nullptr, SPK_ERR_ARG, "%s: null solver", __func__)
+#define SPK_CHECK_SOLVER(s)                                                              \
+    do {                                                                                 \
+        SPK_REQUIRE((s) != nullptr, SPK_ERR_ARG, "%s: null solver", __func__);           \
+        (s)->fused_w_ok = false;                                                         \
+    } while (0)
 
 extern "C" int spk_solver_create(spk_ctx ctx, const spk_params *p, int bulk, spk_solver *out) {
     SPK_REQUIRE(ctx && p && out, SPK_ERR_ARG, "spk_solver_create: null argument");
@@ -1675,7 +1686,7 @@ extern "C" int spk_solver_destroy(spk_solver s) {
 }
 
 extern "C" int spk_solver_info(spk_solver s, int *vars, size_t *ntc) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     if (vars) {
         const int v[10] = {s->VX, s->VY, s->VZ, s->BX, s->BY, s->BZ, s->TH, s->VEL, s->MAG, s->nf};
         memcpy(vars, v, sizeof(v));
@@ -1693,7 +1704,7 @@ extern "C" int spk_solver_buffers(spk_solver s, void **fields, void **dfields, v
 }
 
 extern "C" int spk_solver_get_time(spk_solver s, double *out8) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(out8, SPK_ERR_ARG, "spk_solver_get_time: null out");
     TimeState h;
     SPK_TRY(read_time(s, &h));
@@ -1718,7 +1729,7 @@ extern "C" int spk_clean_divergence(spk_solver s) {
 }
 
 extern "C" int spk_complex2real(spk_solver s) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     const Grid3 &g = s->g;
     PassArgs a{s->U, s->realbuf, s->nf, g.ntc, g.ntc, 1.0 / ((double)g.nx * g.ny * g.nz)};
     SPK_TRY(launch_pass_x(s->ctx, g, s->tw64, +1, SPK_F64, SPK_F64, a));
@@ -1755,9 +1766,11 @@ extern "C" int spk_hydro_mhd_advance(spk_solver s) {
     return hydro_mhd_advance(s);
 }
 
-extern "C" int spk_main_loop(spk_solver s, int nsteps) {
-    SPK_CHECK_SOLVER(s);
-    SPK_REQUIRE(nsteps >= 0, SPK_ERR_ARG, "spk_main_loop: negative step count");
+namespace {
+// resume: the first step may start from the inverse x pass the previous fused
+// step left in W (spk_main_loop_resume); otherwise every call starts from a fresh
+// projection and inverse x pass
+int main_loop(spk_solver s, int nsteps, bool resume) {
     if (!s->loop_started) {
         // supervisor.cu:171-175
         if (s->p.shearing) {
@@ -1773,7 +1786,7 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
         if (fused) {
             // forcing changes the state after the fused back end made the next step's
             // inverse x pass, so that pass is made again from the state
-            SPK_TRY(step_fused(s, n > 0 && s->force_n == 0 && s->fused_w_ok));
+            SPK_TRY(step_fused(s, (n > 0 || resume) && s->force_n == 0 && s->fused_w_ok));
         } else {
             SPK_TRY(clean_divergence(s));
             s->products_valid = false;
@@ -1788,6 +1801,19 @@ extern "C" int spk_main_loop(spk_solver s, int nsteps) {
         SPK_TRY(clock(s, 0, 0, 1, 0, 0.0));
     }
     return SPK_OK;
+}
+}  // namespace
+
+extern "C" int spk_main_loop(spk_solver s, int nsteps) {
+    SPK_CHECK_SOLVER(s);
+    SPK_REQUIRE(nsteps >= 0, SPK_ERR_ARG, "spk_main_loop: negative step count");
+    return main_loop(s, nsteps, false);
+}
+
+extern "C" int spk_main_loop_resume(spk_solver s, int nsteps) {
+    SPK_REQUIRE(s != nullptr, SPK_ERR_ARG, "spk_main_loop_resume: null solver");
+    SPK_REQUIRE(nsteps >= 0, SPK_ERR_ARG, "spk_main_loop_resume: negative step count");
+    return main_loop(s, nsteps, true);
 }
 
 extern "C" int spk_solver_set_forcing(spk_solver s, int nmodes, const long long *index, const double *values) {
@@ -1817,7 +1843,7 @@ extern "C" int spk_solver_set_forcing(spk_solver s, int nmodes, const long long 
 }
 
 extern "C" int spk_energy(spk_solver s, int field, double *out) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(field >= 0 && field < s->nf && out, SPK_ERR_ARG, "spk_energy: bad field %d", field);
     const unsigned nb = red_grid(s->g.ntc);
     k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(s->U + (long long)field * s->g.ntc, s->g.ntc, s->kg0, 0, s->part);
@@ -1827,7 +1853,7 @@ extern "C" int spk_energy(spk_solver s, int field, double *out) {
 }
 
 extern "C" int spk_enstrophy(spk_solver s, int f0, double *out) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(f0 >= 0 && f0 + 2 < s->nf && out, SPK_ERR_ARG, "spk_enstrophy: bad field %d", f0);
     const unsigned nb = red_grid(s->g.ntc);
     k_spec_sum<<<nb, 256, 0, s->ctx->stream>>>(s->U + (long long)f0 * s->g.ntc, s->g.ntc,
@@ -1838,7 +1864,7 @@ extern "C" int spk_enstrophy(spk_solver s, int f0, double *out) {
 }
 
 extern "C" int spk_correlation(spk_solver s, int fa, int fb, double *out) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out, SPK_ERR_ARG, "spk_correlation: bad fields");
     const long long rows = (long long)s->g.nx * s->g.ny;
     const unsigned nb = red_grid(rows * s->g.nz);
@@ -1870,7 +1896,7 @@ int spectrum_of(spk_solver s, const cpx<double> *a, const cpx<double> *b, double
 }  // namespace
 
 extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, int *nbins) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(fa >= 0 && fa < s->nf && fb >= 0 && fb < s->nf && out && cap >= 0, SPK_ERR_ARG,
                 "spk_spectrum: bad arguments");
     if (nbins) *nbins = s->nbins;
@@ -1878,7 +1904,7 @@ extern "C" int spk_spectrum(spk_solver s, int fa, int fb, double *out, int cap, 
 }
 
 extern "C" int spk_spectrum_bins(spk_solver s, int *nbins, double *deltak) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     if (nbins) *nbins = s->nbins;
     if (deltak) *deltak = s->deltak;
     return SPK_OK;
@@ -1943,7 +1969,7 @@ int diag_spec_sum(spk_solver s, const cpx<double> *u, int kind, const KGeom &kg,
 
 // WriteTimevarOutput (spooky_output_timevar.cu:48-204), one quantity by name
 extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(name && out, SPK_ERR_ARG, "spk_timevar: null argument");
     const std::string v(name);
     const spk_params &p = s->p;
@@ -2061,7 +2087,7 @@ extern "C" int spk_timevar(spk_solver s, const char *name, double *out) {
 
 // WriteSpectrumOutput (spooky_output_spectrum.cu:22-400), one spectrum by name
 extern "C" int spk_spectrum_named(spk_solver s, const char *name, double *out, int cap, int *nbins) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(name && out && cap >= 0, SPK_ERR_ARG, "spk_spectrum_named: bad arguments");
     if (nbins) *nbins = s->nbins;
     const std::string v(name);
@@ -2178,7 +2204,7 @@ extern "C" int spk_spectrum_named(spk_solver s, const char *name, double *out, i
 // WriteUserTimevarOutput (user_output_timevar.cu:18-76): the user quantities the
 // reference ships
 extern "C" int spk_user_timevar(spk_solver s, const char *name, double *out) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(name && out, SPK_ERR_ARG, "spk_user_timevar: null argument");
     const std::string v(name);
     const spk_params &p = s->p;
@@ -2214,7 +2240,7 @@ extern "C" int spk_user_timevar(spk_solver s, const char *name, double *out) {
 // of the state, unsheared first in a shearing box (UnshearOutput,
 // unshear_output.cu:21-54: phase exp(-i ky c / 2), c = Lx S fmod(t, 2 Ly / (S Lx)))
 extern "C" int spk_snapshot_real(spk_solver s) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     const spk_params &p = s->p;
     if (!p.shearing) return spk_complex2real(s);
     const Grid3 &g = s->g;
@@ -2332,6 +2358,7 @@ extern "C" int spk_kernel_timings(spk_solver s, int reps, double *out_ms, int ca
 
 namespace spk {
 void solver_views(spk_solver s, SolverViews *v) {
+    s->fused_w_ok = false;  // the caller may change the state through the views
     v->U = s->U;
     v->realbuf = s->realbuf;
     v->ts = s->ts;
@@ -2431,7 +2458,7 @@ extern "C" int spk_solver_set_rkl_device(spk_solver s, int on) {
 }
 
 extern "C" int spk_solver_rkl_plan(spk_solver s, int *stages, int *overflow, int *smax) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     SPK_REQUIRE(s->rkl_plan, SPK_ERR_ARG, "spk_solver_rkl_plan: no super-time-stepping in this run");
     int h[2];
     SPK_HIP(hipMemcpyAsync(h, s->rkl_plan, sizeof(h), hipMemcpyDeviceToHost, s->ctx->stream));
@@ -2466,14 +2493,14 @@ extern "C" int spk_solver_set_plane(spk_solver s, int on) {
 }
 
 extern "C" int spk_solver_plane(spk_solver s, int *rk3, int *rkl2) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     if (rk3) *rk3 = plane_front(s) ? 1 : 0;
     if (rkl2) *rkl2 = plane_rkl(s) && s->rkl_aniso && rkl_fused(s) ? 1 : 0;
     return SPK_OK;
 }
 
 extern "C" int spk_solver_pipeline(spk_solver s, int *fused, int *points) {
-    SPK_CHECK_SOLVER(s);
+    SPK_CHECK_SOLVER_RO(s);
     if (fused) *fused = fused_ok(s) ? 1 : 0;
     if (points) *points = s->fused_ef;
     return SPK_OK;

@@ -332,7 +332,9 @@ int main(int argc, char **argv) {
     long nsteps = 0;
     while (R.tm[0] < p.t_final && elapsed < R.rp.max_walltime_elapsed * 3600.0) {
         const double lastsnap = R.tm[6];
-        CALL(spk_main_loop(R.sol, 1));
+        // the first step starts fresh; later ones continue from the last step's
+        // inverse x pass (the outputs between steps only read the state)
+        CALL(nsteps == 0 ? spk_main_loop(R.sol, 1) : spk_main_loop_resume(R.sol, 1));
         R.read_time();
         nsteps++;
         R.check_output(R.tm[6] != lastsnap);

@@ -225,6 +225,49 @@ def test_fused_path_matches_unfused_and_oracle(ctx, name, prec, points):
     assert t["fwdx_epilogue"] is not None and t["fwdx_epilogue"] > 0
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["mhd_x32", "hd_strat_x32", "mhd_bouss_rkl2_x32", "hd_strat_sts_u_x32",
+                                  "q2d_hbi_y512"])
+def test_main_loop_resume(ctx, name, prec):
+    """spk_main_loop_resume continues the previous call: one step and then four
+    resumed single steps give the state and clock of one five-step call, bit for
+    bit (super-time-stepped cases cross both cycle placements); after another
+    entry point (compute_dfield, which rewrites the transform buffers) a resumed
+    call starts fresh, as a plain call does."""
+    p = CASES[name]["params"]
+    u0 = initial_state(name)
+    sols = []
+    for _ in range(4):
+        sol = spk.Solver(ctx, prec, **p)
+        sol.set_fused(2)
+        sol.set_fields(u0)
+        sols.append(sol)
+    a, b, c, d = sols
+    a.main_loop(5)
+    b.main_loop(1)
+    for _ in range(4):
+        b.main_loop(1, resume=True)
+    assert np.array_equal(a.get_fields(), b.get_fields())
+    assert a.time() == b.time()
+    c.main_loop(1)
+    c.compute_dfield()
+    c.main_loop(1, resume=True)
+    d.main_loop(1)
+    d.main_loop(1)
+    assert np.array_equal(c.get_fields(), d.get_fields())
+    # resuming from a changed state (set_fields) starts fresh too
+    a.set_fields(u0)
+    a.set_time(0.0)
+    a.main_loop(1, resume=True)
+    e = spk.Solver(ctx, prec, **p)
+    e.set_fused(2)
+    e.set_fields(u0)
+    e.main_loop(1)
+    assert np.array_equal(a.get_fields(), e.get_fields())
+    for s_ in sols + [e]:
+        s_.destroy()
+
+
 @pytest.mark.parametrize("fused", [2, 0])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("name", sorted(PLANE_CASES))
