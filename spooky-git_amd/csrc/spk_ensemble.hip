@@ -17,7 +17,8 @@
 //
 // Two steps of every box are captured once into one HIP graph (fork from an
 // origin stream into the lanes, join back) and each replay is a single graph
-// launch.  A captured step needs no host synchronization: dt stays on the
+// launch; a second graph of the same steps continues from the inverse x pass
+// the previous replay left (the fresh one projects and transforms first).  A captured step needs no host synchronization: dt stays on the
 // device and RKL2 cycles are planned on the device (spk_solver_set_rkl_device),
 // every box with its own stage count (stages past a box's count return at once).
 #include "spk_internal.hpp"
@@ -53,6 +54,10 @@ struct spk_ensemble_s {
     hipEvent_t ev_fork = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the same two steps continuing from the inverse x pass the previous replay
+    // left in W (spk_main_loop_resume), where every box's steps leave one
+    hipGraph_t graph_warm = nullptr;
+    hipGraphExec_t exec_warm = nullptr;
     std::vector<int> parity;          // host step parity of each box when the graph was captured
 };
 
@@ -65,9 +70,16 @@ constexpr int MAX_LANES = 16;
 void drop_graph(spk_ensemble e) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->exec_warm) (void)hipGraphExecDestroy(e->exec_warm);
+    if (e->graph_warm) (void)hipGraphDestroy(e->graph_warm);
     e->exec = nullptr;
     e->graph = nullptr;
+    e->exec_warm = nullptr;
+    e->graph_warm = nullptr;
 }
+
+// two steps of one solver: a fresh start, or continuing from W
+int two_steps(spk_solver s, bool resume) { return resume ? spk_main_loop_resume(s, 2) : spk_main_loop(s, 2); }
 
 void drop_batches(spk_ensemble e) {
     for (BoxBatch &b : e->batches)
@@ -93,12 +105,12 @@ int fork_origin_into_lanes(spk_ensemble e) {
 // Two steps of the boxes of one batch: one launch per kernel for all of them,
 // through the leader's context.  Boxes whose host step differs from the
 // leader's (one was stepped on its own) run box by box instead.
-int batch_two_steps(spk_ensemble e, const BoxBatch &b) {
+int batch_two_steps(spk_ensemble e, const BoxBatch &b, bool resume) {
     spk_solver lead = e->box[b.boxes[0]];
     bool aligned = true;
     for (int i : b.boxes) aligned = aligned && solver_host_step(e->box[i]) == solver_host_step(lead);
     if (!aligned || b.boxes.size() == 1) {
-        for (int i : b.boxes) SPK_TRY(spk_main_loop(e->box[i], 2));
+        for (int i : b.boxes) SPK_TRY(two_steps(e->box[i], resume));
         return SPK_OK;
     }
     spk_ctx c = solver_ctx(lead);
@@ -108,7 +120,7 @@ int batch_two_steps(spk_ensemble e, const BoxBatch &b) {
         c->base[k] = b.base[k];
         c->size[k] = b.size[k];
     }
-    const int rc = spk_main_loop(lead, 2);
+    const int rc = two_steps(lead, resume);
     c->rows = nullptr;
     c->nbox = 1;
     SPK_TRY(rc);
@@ -117,20 +129,31 @@ int batch_two_steps(spk_ensemble e, const BoxBatch &b) {
 }
 
 // Two steps of every box: batch by batch, or box by box over the lanes.
-int issue_two_steps(spk_ensemble e) {
+int issue_two_steps(spk_ensemble e, bool resume) {
     if (e->split == 0) {
-        for (spk_solver s : e->box) SPK_TRY(spk_main_loop(s, 2));
+        for (spk_solver s : e->box) SPK_TRY(two_steps(s, resume));
         return SPK_OK;
     }
-    for (const BoxBatch &b : e->batches) SPK_TRY(batch_two_steps(e, b));
+    for (const BoxBatch &b : e->batches) SPK_TRY(batch_two_steps(e, b, resume));
     return SPK_OK;
 }
 
-int capture(spk_ensemble e) {
-    drop_graph(e);
+// every box's fused steps leave the next step's inverse x pass in W
+bool all_continue(spk_ensemble e) {
+    for (spk_solver s : e->box)
+        if (!solver_continues(s)) return false;
+    return true;
+}
+bool all_w_ok(spk_ensemble e) {
+    for (spk_solver s : e->box)
+        if (!solver_w_ok(s)) return false;
+    return true;
+}
+
+int capture_one(spk_ensemble e, bool resume, hipGraph_t *graph, hipGraphExec_t *exec) {
     SPK_HIP(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
     int rc = fork_origin_into_lanes(e);
-    if (rc == SPK_OK) rc = issue_two_steps(e);
+    if (rc == SPK_OK) rc = issue_two_steps(e, resume);
     if (rc == SPK_OK) rc = join_lanes_into_origin(e);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(e->origin, &g);
@@ -145,13 +168,32 @@ int capture(spk_ensemble e) {
         set_error("spk_ensemble: graph capture failed: %s", hipGetErrorString(ec));
         return SPK_ERR_HIP;
     }
-    e->graph = g;
-    const hipError_t ei = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0);
+    *graph = g;
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
-        e->exec = nullptr;
-        drop_graph(e);
+        *exec = nullptr;
         set_error("spk_ensemble: graph instantiation failed: %s", hipGetErrorString(ei));
         return SPK_ERR_HIP;
+    }
+    return SPK_OK;
+}
+
+// The fresh-start graph, and where every box continues from step to step the
+// graph that continues from the previous replay's W.  Capturing runs the step
+// logic on the host only, so each box's record of W is kept across it.
+int capture(spk_ensemble e) {
+    drop_graph(e);
+    std::vector<char> wok(e->box.size());
+    for (size_t i = 0; i < e->box.size(); i++) wok[i] = solver_w_ok(e->box[i]);
+    int rc = capture_one(e, false, &e->graph, &e->exec);
+    if (rc == SPK_OK && all_continue(e)) {
+        for (spk_solver s : e->box) solver_set_w_ok(s, true);
+        rc = capture_one(e, true, &e->graph_warm, &e->exec_warm);
+    }
+    for (size_t i = 0; i < e->box.size(); i++) solver_set_w_ok(e->box[i], wok[i] != 0);
+    if (rc != SPK_OK) {
+        drop_graph(e);
+        return rc;
     }
     e->parity.resize(e->box.size());
     for (size_t i = 0; i < e->box.size(); i++) e->parity[i] = solver_host_step(e->box[i]) & 1;
@@ -338,8 +380,13 @@ extern "C" int spk_ensemble_run(spk_ensemble e, int nsteps) {
     SPK_REQUIRE(nsteps >= 0 && nsteps % 2 == 0, SPK_ERR_ARG,
                 "spk_ensemble_run: %d steps (an even count: a replay advances every box by two)", nsteps);
     if (nsteps == 0) return SPK_OK;
+    // every box continues from the inverse x pass its previous steps left in W
+    // (no box was touched since), or the first two steps start fresh
+    const bool cont = all_continue(e);
+    const bool warm = cont && all_w_ok(e);
     if (!e->use_graph) {
-        for (int r = 0; r < nsteps / 2; r++) SPK_TRY(issue_two_steps(e));
+        for (int r = 0; r < nsteps / 2; r++) SPK_TRY(issue_two_steps(e, r > 0 ? cont : warm));
+        for (spk_solver s : e->box) solver_set_w_ok(s, cont);
         return SPK_OK;
     }
     bool stale = e->exec == nullptr;
@@ -347,9 +394,15 @@ extern "C" int spk_ensemble_run(spk_ensemble e, int nsteps) {
         stale = (solver_host_step(e->box[i]) & 1) != e->parity[i];
     if (stale) SPK_TRY(capture(e));
     SPK_TRY(join_lanes_into_origin(e));
-    for (int r = 0; r < nsteps / 2; r++) SPK_HIP(hipGraphLaunch(e->exec, e->origin));
+    for (int r = 0; r < nsteps / 2; r++) {
+        const bool w = e->exec_warm != nullptr && (r > 0 || warm);
+        SPK_HIP(hipGraphLaunch(w ? e->exec_warm : e->exec, e->origin));
+    }
     SPK_TRY(fork_origin_into_lanes(e));
-    for (spk_solver s : e->box) solver_advance_host(s, nsteps);
+    for (spk_solver s : e->box) {
+        solver_advance_host(s, nsteps);
+        solver_set_w_ok(s, e->exec_warm != nullptr);
+    }
     return SPK_OK;
 }
 

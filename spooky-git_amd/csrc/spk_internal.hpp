@@ -161,5 +161,12 @@ void solver_box_row(spk_solver s, BoxRow *row, size_t *sizes);
 bool solver_same_shape(spk_solver a, spk_solver b);
 // s takes the back end (fused or not, points per thread) that lead runs
 int solver_follow_back_end(spk_solver s, spk_solver lead);
+// whether W holds the inverse x pass the last fused step of s left (the host's
+// record of it), and setting that record (for steps replayed from a graph)
+bool solver_w_ok(spk_solver s);
+void solver_set_w_ok(spk_solver s, bool v);
+// a fused step of s leaves W holding the next step's inverse x pass (fused back
+// end, no forcing, no super-time-stepped u or B)
+bool solver_continues(spk_solver s);
 
 }  // namespace spk

@@ -2430,6 +2430,13 @@ bool solver_same_shape(spk_solver a, spk_solver b) {
            (a->Breal != nullptr) == (b->Breal != nullptr) && a->plane_on == b->plane_on;
 }
 
+bool solver_w_ok(spk_solver s) { return s->fused_w_ok; }
+void solver_set_w_ok(spk_solver s, bool v) { s->fused_w_ok = v; }
+bool solver_continues(spk_solver s) {
+    const bool sts_uv = s->p.supertimestepping && s->nsts > 0 && !(s->nsts == 1 && s->sts_index[0] == s->TH);
+    return fused_ok(s) && s->force_n == 0 && !sts_uv;
+}
+
 int solver_follow_back_end(spk_solver s, spk_solver lead) {
     s->fused_mode = lead->fused_mode;
     s->fused_faster = lead->fused_faster;

@@ -179,6 +179,32 @@ def test_batched_launches_match_box_by_box(ctx, prec):
     assert t3["dt_sts"] != t4["dt_sts"]
 
 
+def test_ensemble_restarts_fresh_after_new_state(ctx):
+    """Replays continue from the inverse x pass the previous replay left; new
+    states set between runs make the next replay start fresh, so an ensemble
+    stepped, reloaded and stepped again equals a new ensemble stepped from the
+    reloaded states, bit for bit."""
+    bx = boxes()
+    ens = spk.Ensemble(bx, "f64", lanes=2)
+    for i, p in enumerate(bx):
+        ens.set_fields(i, box_state(p, 700 + i))
+    ens.run(4)
+    new = [box_state(p, 800 + i) for i, p in enumerate(bx)]
+    for i, st in enumerate(new):
+        ens.set_fields(i, st)
+        ens.boxes[i].set_time(0.0)
+    ens.run(4)
+    ref = spk.Ensemble(bx, "f64", lanes=2)
+    for i, st in enumerate(new):
+        ref.set_fields(i, st)
+    ref.run(4)
+    for i in range(len(bx)):
+        assert np.array_equal(ens.get_fields(i), ref.get_fields(i)), i
+        assert ens.time(i) == ref.time(i), i
+    ens.destroy()
+    ref.destroy()
+
+
 def test_ensemble_tune_restores_state(ctx):
     bx = boxes()
     states = [box_state(p, 300 + i) for i, p in enumerate(bx)]
@@ -194,14 +220,19 @@ def test_ensemble_tune_restores_state(ctx):
         assert np.array_equal(ens.get_fields(i), before[i])
         t = ens.time(i)
         assert t["t"] == tb[i]["t"] and t["step"] == tb[i]["step"]
-    # and it keeps stepping from there as an untuned ensemble would
+    # and it keeps stepping from there as an untuned ensemble would after its
+    # states were set again (the restored states start afresh: the probes
+    # overwrote the transform buffers the next replay would otherwise continue from)
     ens.run(2)
     after = ens.get_fields(3)
     ens.destroy()
     ref = spk.Ensemble(bx, "f32", lanes=1)
     for i, st in enumerate(states):
         ref.set_fields(i, st)
-    ref.run(4)
+    ref.run(2)
+    for i in range(len(bx)):
+        ref.set_fields(i, ref.get_fields(i))
+    ref.run(2)
     assert np.max(np.abs(ref.get_fields(3) - after)) <= 1e-14 * np.max(np.abs(after))
     ref.destroy()
 
