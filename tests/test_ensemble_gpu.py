@@ -185,7 +185,15 @@ def test_ensemble_restarts_fresh_after_new_state(ctx):
     stepped, reloaded and stepped again equals a new ensemble stepped from the
     reloaded states, bit for bit."""
     bx = boxes()
-    ens = spk.Ensemble(bx, "f64", lanes=2)
+
+    def pinned(e):
+        # the same back end and tile in both ensembles (each creation autotunes its own)
+        for s_ in e.boxes:
+            s_.set_fused(2)
+            s_.set_fused_points(8)
+        return e
+
+    ens = pinned(spk.Ensemble(bx, "f64", lanes=2))
     for i, p in enumerate(bx):
         ens.set_fields(i, box_state(p, 700 + i))
     ens.run(4)
@@ -194,7 +202,7 @@ def test_ensemble_restarts_fresh_after_new_state(ctx):
         ens.set_fields(i, st)
         ens.boxes[i].set_time(0.0)
     ens.run(4)
-    ref = spk.Ensemble(bx, "f64", lanes=2)
+    ref = pinned(spk.Ensemble(bx, "f64", lanes=2))
     for i, st in enumerate(new):
         ref.set_fields(i, st)
     ref.run(4)
@@ -209,6 +217,9 @@ def test_ensemble_tune_restores_state(ctx):
     bx = boxes()
     states = [box_state(p, 300 + i) for i, p in enumerate(bx)]
     ens = spk.Ensemble(bx, "f32")
+    for s_ in ens.boxes:  # the same back end and tile as the reference ensemble below
+        s_.set_fused(2)
+        s_.set_fused_points(8)
     for i, st in enumerate(states):
         ens.set_fields(i, st)
     ens.run(2)
@@ -227,6 +238,9 @@ def test_ensemble_tune_restores_state(ctx):
     after = ens.get_fields(3)
     ens.destroy()
     ref = spk.Ensemble(bx, "f32", lanes=1)
+    for s_ in ref.boxes:
+        s_.set_fused(2)
+        s_.set_fused_points(8)
     for i, st in enumerate(states):
         ref.set_fields(i, st)
     ref.run(2)
